@@ -1,0 +1,248 @@
+"""Benchmark: TFLOP/s at FP32 accuracy, split-BF16 vs plain FP32 MFMA, on one MI355X.
+
+A step is one C = A*B call through the C ABI (maxmulsk_sgemm_ex, device
+pointers, inputs resident in HBM) at M = N = K = 4096 in the split-BF16
+3-product mode. The same run also times the plain FP32-MFMA path at 4096^3 and
+both paths on the narrow-M shapes (M x N x K), and reports each result's error
+against an FP64 product.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TFLOP/s at FP32 accuracy: split-BF16 vs plain FP32 MFMA (4096³, M=64 shapes)"
+U32 = 2.0 ** -24
+BF16_PEAK = 2516.6   # TFLOP/s, MI355X dense BF16 MFMA
+FP32_PEAK = 157.3    # TFLOP/s, MI355X FP32 MFMA
+HBM_PEAK = 8000.0    # GB/s
+NARROW = [(64, 512, 8192), (64, 24576, 1536), (64, 32768, 512), (128, 512, 32768), (64, 7168, 16384)]
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--only", default="", help="comma list: main,narrow (default all)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the GEMM path has no "
+              "multi-GPU mode in this build.", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import maxmulsk_amd as mm
+    from maxmulsk_amd import inputs
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    if mm.device_count() < 1:
+        raise SystemExit("libmaxmulsk sees no gfx950 device: " + mm.last_error())
+    dev = torch.device("cuda")
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+    only = set(args.only.split(",")) if args.only else {"main", "narrow"}
+
+    def upload(x):
+        return torch.from_numpy(x).to(dev)
+
+    def call(dA, dB, dC, M, K, N, mode, plan=None, ws=None):
+        ws_ptr = ws.data_ptr() if ws is not None else None
+        ws_bytes = ws.numel() * 4 if ws is not None else 0
+        st = mm.sgemm_ex(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, mode=mode, plan=plan,
+                         stream=sptr, async_=True, workspace=ws_ptr, workspace_bytes=ws_bytes)
+        if st != mm.OK:
+            raise RuntimeError(f"sgemm_ex status {st}: {mm.last_error()}")
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        wall = time.perf_counter()
+        t0.record(stream)
+        for _ in range(steps):
+            fn()
+        t1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - wall
+        return t0.elapsed_time(t1) / steps, wall
+
+    def err_fp64(dA, dB, dC, K):
+        A64, B64 = dA.double(), dB.double()
+        C64 = A64 @ B64
+        AB = A64.abs() @ B64.abs()
+        d = (dC.double() - C64).abs()
+        ratio = float((d / (K * U32 * AB.clamp_min(1e-300))).max())
+        max_rel = float((d / torch.maximum(C64.abs(), U32 * AB).clamp_min(1e-300)).max())
+        rel_frob = float(d.norm() / C64.norm())
+        return {"bound_ratio": ratio, "max_rel": max_rel, "rel_frob": rel_frob,
+                "within_fp32_bound": ratio <= 1.0}
+
+    def workspace(M, K, N, mode, plan):
+        nbytes = mm.workspace_bytes(M, K, N, mode, plan)
+        return torch.empty(max(nbytes // 4, 1) + 64, dtype=torch.float32, device=dev) if nbytes else None
+
+    out: dict = {}
+    shapes = []
+    ms_step = None
+    main_plan = None
+
+    if "main" in only:
+        M = N = K = 4096
+        log("generating 4096^3 inputs")
+        dA, dB = upload(inputs.uniform(1, (M, K))), upload(inputs.uniform(2, (K, N)))
+        dC = torch.empty((M, N), dtype=torch.float32, device=dev)
+        res = {}
+        for mode_name, mode in (("plain", 0), ("split3", 1), ("split6", 2)):
+            plan, src = mm.choose_plan(M, K, N, mode)
+            ws = workspace(M, K, N, mode, plan)
+            steps = args.steps
+            ms, wall = timed(lambda: call(dA, dB, dC, M, K, N, mode, plan, ws), steps, args.warmup)
+            e = err_fp64(dA, dB, dC, K)
+            tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+            res[mode_name] = {"tflops": tf, "ms": ms, "plan": list(plan), "errors": e}
+            log(f"4096^3 {mode_name}: {ms:.4f} ms  {tf:.1f} TFLOP/s  plan {plan}  err {e}")
+            if mode_name == "split3":
+                ms_step, main_plan, main_wall = ms, plan, wall
+        out["main"] = res
+        # the headline: the fastest split mode that stays inside the FP32 bound
+        ok = [m for m in ("split3", "split6") if res[m]["errors"]["within_fp32_bound"]]
+        head = max(ok, key=lambda m: res[m]["tflops"]) if ok else "split3"
+        out["headline_mode"] = head
+        del dA, dB, dC
+        torch.cuda.empty_cache()
+
+    if "narrow" in only:
+        for (M, N, K) in NARROW:
+            A = inputs.uniform(3, (M, K))
+            B = inputs.uniform(4, (K, N))
+            dA, dB = upload(A), upload(B)
+            dC = torch.empty((M, N), dtype=torch.float32, device=dev)
+            row = {"shape_mnk": [M, N, K]}
+            for mode_name, mode in (("plain", 0), ("split3", 1)):
+                plan, _ = mm.choose_plan(M, K, N, mode)
+                ws = workspace(M, K, N, mode, plan)
+                ms, _ = timed(lambda: call(dA, dB, dC, M, K, N, mode, plan, ws), max(args.steps, 20), args.warmup)
+                e = err_fp64(dA, dB, dC, K)
+                flops = 2.0 * M * N * K
+                alg_bytes = 4.0 * (M * K + K * N + M * N)
+                row[mode_name] = {"tflops": flops / (ms * 1e-3) / 1e12, "ms": ms, "plan": list(plan),
+                                  "hbm_gbs": alg_bytes / (ms * 1e-3) / 1e9, "errors": e}
+                log(f"{M}x{N}x{K} {mode_name}: {ms:.4f} ms {row[mode_name]['tflops']:.1f} TFLOP/s "
+                    f"{row[mode_name]['hbm_gbs']:.0f} GB/s plan {plan} ratio {e['bound_ratio']:.3g}")
+            shapes.append(row)
+            del dA, dB, dC
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.skip_cpu:
+        cpu = cpu_baseline(args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "data": "synthetic: U[-1,1) FP32 from splitmix64 seeds (A seed 1, B seed 2; narrow shapes 3, 4)",
+    }
+    if "main" in out:
+        head = out["headline_mode"]
+        r = out["main"][head]
+        line["value"] = r["tflops"]
+        line["ms_per_step"] = r["ms"]
+        line["dtype"] = "fp32 in/out; bf16 " + ("3-term" if head == "split3" else "6-term") + " split MFMA, fp32 accumulate"
+        line["config"] = {"workload": f"split-BF16 GEMM C=A*B, M=N=K=4096, mode {head}, FP32 inputs/outputs",
+                          "M": 4096, "N": 4096, "K": 4096, "mode": head, "plan": r["plan"]}
+        line["plain_fp32_mfma_tflops"] = out["main"]["plain"]["tflops"]
+        line["speedup_vs_plain"] = r["tflops"] / out["main"]["plain"]["tflops"]
+        line["modes_4096"] = out["main"]
+        # Dominant kernel: the split GEMM kernel (one launch per call when the
+        # plan has no split-K). Issued BF16 MFMA FLOPs = products * 2MNK.
+        products = 3 if head == "split3" else 6
+        issued = products * 2.0 * 4096 ** 3
+        line["roofline"] = {"bound": "mfma", "achieved": issued / (r["ms"] * 1e-3) / 1e12,
+                            "peak": BF16_PEAK, "unit": "TFLOP/s",
+                            "frac": issued / (r["ms"] * 1e-3) / 1e12 / BF16_PEAK, "traffic": None,
+                            "kernel_ms": r["ms"],
+                            "note": f"issued BF16 MFMA FLOPs = {products} x 2MNK per launch"}
+        line["plain_roofline"] = {"bound": "mfma", "achieved": out["main"]["plain"]["tflops"],
+                                  "peak": FP32_PEAK, "unit": "TFLOP/s",
+                                  "frac": out["main"]["plain"]["tflops"] / FP32_PEAK}
+    if shapes:
+        line["narrow_m"] = shapes
+    if cpu:
+        line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline(seconds: float) -> dict:
+    """The oracle's restatement of the reference's portable FP32 loop
+    (Utils::multiply_scalar), single thread, at 256^3; plus the host BLAS
+    (numpy float32 matmul) at 256^3 and 4096^3 for context."""
+    from oracle import oracle as orc
+
+    n = 256
+    A = orc.uniform(1, (n, n))
+    B = orc.uniform(2, (n, n))
+    orc.sgemm_portable(A, B)  # warm
+    reps, t0 = 0, time.perf_counter()
+    samples = []
+    while time.perf_counter() - t0 < seconds:
+        t = time.perf_counter()
+        orc.sgemm_portable(A, B)
+        samples.append(time.perf_counter() - t)
+        reps += 1
+    med = float(np.median(samples))
+    flops = 2.0 * n ** 3
+    res = {"value": flops / med / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+           "sample": f"oracle restatement of Utils::multiply_scalar (common/utils.hpp:50-61), "
+                     f"M=N=K=256, {reps} calls in {time.perf_counter() - t0:.1f} s, median",
+           "gflops": flops / med / 1e9}
+    try:
+        info = np.__config__.CONFIG["Build Dependencies"]["blas"]
+        blas = f"{info.get('name')} {info.get('version')}"
+    except Exception:  # pragma: no cover - informational only
+        blas = "numpy BLAS"
+    blas_rows = {}
+    for m in (256, 4096):
+        a = orc.uniform(5, (m, m))
+        b = orc.uniform(6, (m, m))
+        a @ b
+        ts = []
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < min(3.0, seconds) or len(ts) < 3:
+            t = time.perf_counter()
+            a @ b
+            ts.append(time.perf_counter() - t)
+        blas_rows[f"{m}^3"] = 2.0 * m ** 3 / float(np.median(ts)) / 1e12
+    res["host_blas_sgemm_tflops"] = blas_rows
+    res["host_blas"] = f"{blas}, all {os.cpu_count()} threads"
+    return res
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,221 @@
+#pragma once
+
+#include <stddef.h>
+
+/*
+ * MaxMulSK for AMD Instinct MI355X (gfx950).
+ *
+ * The first two entry points keep the reference library's C ABI byte for byte
+ * (capi/maxmulsk.h:24-37): C = A*B for row-major, tightly packed FP32 matrices,
+ * A is M x K, B is K x N, C is M x N, C is overwritten (alpha = 1, beta = 0) and
+ * its previous contents are never read. Dimensions are in (M, K, N) order, not
+ * BLAS's (M, N, K).
+ *
+ * Pointers may be host memory (staged through device buffers, a true drop-in for
+ * the CPU library) or device memory from hipMalloc (zero-copy). Both entry points
+ * are synchronous: they return with C complete. When no usable gfx950 device is
+ * present they return MAXMULSK_ERR_UNSUPPORTED and leave C untouched instead of
+ * crashing (the reference dies with SIGILL without SME, capi/maxmulsk.h:16-17).
+ *
+ * Everything after maxmulsk_sgemm_v4 is an additive extension with no
+ * counterpart in the reference's C ABI unless its comment names one.
+ */
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same names and values as capi/maxmulsk.h:24-28. MAXMULSK_ERR_DEVICE is
+ * additive: a HIP runtime call failed after the arguments were accepted. */
+typedef enum {
+    MAXMULSK_OK = 0,
+    MAXMULSK_ERR_UNSUPPORTED,   /* shape, plan, mode or device rejected; C untouched */
+    MAXMULSK_ERR_ALLOC,         /* a device workspace or staging buffer could not be allocated */
+    MAXMULSK_ERR_DEVICE,        /* a HIP call failed (launch or copy) */
+} maxmulsk_status;
+
+/* Replaces capi/maxmulsk.h:32-33 (maxmulsk_sgemm_v3 -> SMEKernels1x4AccKcOut
+ * with the v3 tuning table, sme/support/gemm_tuning.cpp:81-100). Here: plain
+ * FP32 MFMA GEMM, plan from the fixed shape rule only (plus the MAXMULSK_PLAN
+ * environment override, the analogue of MAXMULSK_BLOCKING). */
+maxmulsk_status maxmulsk_sgemm_v3(const float* A, const float* B, float* C,
+                                  size_t M, size_t K, size_t N);
+
+/* Replaces capi/maxmulsk.h:36-37 (maxmulsk_sgemm_v4 -> the v4c shape rule,
+ * sme/v4/sme-1x4-v4c-dispatch.cpp:17-31). Here: plain FP32 MFMA GEMM, plan from
+ * the autotuner cache, then the learned selector, then the shape rule. */
+maxmulsk_status maxmulsk_sgemm_v4(const float* A, const float* B, float* C,
+                                  size_t M, size_t K, size_t N);
+
+/* ------------------------------------------------------------------------- */
+/* Extensions                                                                */
+/* ------------------------------------------------------------------------- */
+
+typedef enum {
+    MAXMULSK_MODE_PLAIN  = 0,  /* FP32 operands, v_mfma_f32_32x32x2_f32, FP32 accumulate */
+    MAXMULSK_MODE_SPLIT3 = 1,  /* a = hi + lo (BF16), hi*hi + hi*lo + lo*hi on BF16 MFMA */
+    MAXMULSK_MODE_SPLIT6 = 2,  /* a = hi + mid + lo (BF16), six cross products on BF16 MFMA */
+} maxmulsk_mode;
+
+/* A launch plan: output tile edge (64, 128 or 256; the M edge shrinks to 64 or
+ * 128 for narrow M) and the number of K slices (split-K, 1..16). {0, 0} asks
+ * the planner. Analogue of the reference's Blocking {Mc, Nc, Kc}
+ * (sme/v4/sme-1x4-kcout-ncblock-apack4za.hpp:30). */
+typedef struct {
+    int tile;
+    int split_k;
+} maxmulsk_plan;
+
+typedef struct {
+    maxmulsk_mode mode;
+    maxmulsk_plan plan;        /* {0,0}: planner */
+    void* stream;              /* hipStream_t; NULL: the legacy default stream */
+    int async;                 /* 0: return with C complete; 1: stream-ordered (device pointers only) */
+    void* workspace;           /* optional caller-owned device workspace (split-K partials) */
+    size_t workspace_bytes;
+} maxmulsk_options;
+
+/* C = A*B with an explicit mode, plan, stream and workspace. opt may be NULL
+ * (plain mode, planner, default stream, synchronous). */
+maxmulsk_status maxmulsk_sgemm_ex(const float* A, const float* B, float* C,
+                                  size_t M, size_t K, size_t N,
+                                  const maxmulsk_options* opt);
+
+/* Bytes of device workspace a call with this shape, mode and plan needs
+ * (0 when the plan needs none). Counterpart of
+ * SMEKernels1x4FusedPackV51::workspace_bytes (sme/v5/sme-1x4-fusedpack-v51.hpp:72).
+ * plan may be NULL (planner). Returns 0 for an invalid plan. */
+size_t maxmulsk_workspace_bytes(size_t M, size_t K, size_t N, maxmulsk_mode mode,
+                                const maxmulsk_plan* plan);
+
+/* Caller-owned workspace variant: no allocation inside the call when A, B and C
+ * are device pointers. ws must be 256-byte aligned device memory of at least
+ * maxmulsk_workspace_bytes() bytes (NULL allowed when that is 0); its contents
+ * need no initialisation. Counterpart of run_multiplication_ws
+ * (sme/v5/sme-1x4-fusedpack-v51.hpp:76-78); a null, short or misaligned
+ * workspace is MAXMULSK_ERR_ALLOC, as AllocationFailed is there (v51.cpp:734). */
+maxmulsk_status maxmulsk_sgemm_ws(const float* A, const float* B, float* C,
+                                  size_t M, size_t K, size_t N,
+                                  maxmulsk_mode mode, const maxmulsk_plan* plan,
+                                  void* ws, size_t ws_bytes, void* stream);
+
+/* The plan the planner would run for this shape and mode, without running
+ * anything (counterpart of SMEKernels1x4V4C::choose,
+ * sme/v4/sme-1x4-v4c-dispatch.hpp:61). source receives 0 = environment
+ * override, 1 = autotuner cache, 2 = learned selector, 3 = shape rule
+ * (may be NULL). */
+maxmulsk_status maxmulsk_choose_plan(size_t M, size_t K, size_t N, maxmulsk_mode mode,
+                                     maxmulsk_plan* out, int* source);
+
+/* Shape rule only (tier 3 of maxmulsk_choose_plan), pure. */
+maxmulsk_plan maxmulsk_rule_plan(size_t M, size_t K, size_t N, maxmulsk_mode mode);
+
+/* One GEMM of a batch: counterpart of struct Gemm
+ * (sme/v4/sme-1x4-v4c-onestream.hpp:62). */
+typedef struct {
+    const float* A;
+    const float* B;
+    float* C;
+    size_t M, K, N;
+} maxmulsk_gemm;
+
+/* Runs every problem of the batch in ONE persistent launch (plus one split-K
+ * reduce launch when any problem is split). Device pointers only. Every problem
+ * is validated before anything runs; the first failure is returned and no C is
+ * touched (the validate-all-first contract of run_batch,
+ * sme/v4/sme-1x4-v4c-onestream.cpp:683-694). opt->plan is ignored: each problem
+ * takes its own plan from maxmulsk_choose_plan. */
+maxmulsk_status maxmulsk_sgemm_grouped(const maxmulsk_gemm* problems, size_t n,
+                                       const maxmulsk_options* opt);
+
+/* A prepared batch: plans, work list and workspace are built once and the
+ * persistent launch is captured in a hipGraph; each launch replays the graph.
+ * The pointers inside the problems are baked in. */
+typedef struct maxmulsk_grouped_plan_s* maxmulsk_grouped_plan;
+maxmulsk_status maxmulsk_grouped_create(const maxmulsk_gemm* problems, size_t n,
+                                        maxmulsk_mode mode, void* stream,
+                                        maxmulsk_grouped_plan* out);
+maxmulsk_status maxmulsk_grouped_launch(maxmulsk_grouped_plan plan, void* stream);
+/* Work-list statistics: number of tiles (work items) and algorithmic FLOPs. */
+maxmulsk_status maxmulsk_grouped_info(maxmulsk_grouped_plan plan, size_t* work_items,
+                                      double* flops, int* used_graph);
+void maxmulsk_grouped_destroy(maxmulsk_grouped_plan plan);
+
+/* ---- autotuner (two-stage, correctness-gated; counterpart of the blocking
+ * sweep bench/results/2026-09-13/source/bench_nb_sweep.cpp:4-18) ---------- */
+
+/* Number of candidates in the fixed candidate set and the i-th candidate. */
+int maxmulsk_num_candidates(void);
+maxmulsk_plan maxmulsk_candidate(int i);
+
+/* Times every candidate on device buffers A, B, C (device pointers), writes
+ * per-candidate median milliseconds to times_ms[maxmulsk_num_candidates()]
+ * (negative = failed the correctness gate or could not run), stores the best
+ * plan in *best and in the in-process cache. stage1_reps / stage2_reps <= 0
+ * select defaults. */
+maxmulsk_status maxmulsk_autotune(const float* A, const float* B, float* C,
+                                  size_t M, size_t K, size_t N, maxmulsk_mode mode,
+                                  int stage1_reps, int stage2_reps,
+                                  float* times_ms, maxmulsk_plan* best);
+
+/* Cache of autotuned plans, keyed by (M, K, N, mode). */
+maxmulsk_status maxmulsk_cache_put(size_t M, size_t K, size_t N, maxmulsk_mode mode,
+                                   maxmulsk_plan plan);
+void maxmulsk_cache_clear(void);
+size_t maxmulsk_cache_size(void);
+/* Versioned little-endian file with the device name in its header; a file
+ * written for another device or format is refused (MAXMULSK_ERR_UNSUPPORTED). */
+maxmulsk_status maxmulsk_cache_save(const char* path);
+maxmulsk_status maxmulsk_cache_load(const char* path);
+
+/* ---- learned tile selector (3-layer MLP cost model) -------------------- */
+
+/* Network geometry: 4 inputs (log2 M, log2 N, log2 K, mode), two hidden layers
+ * of 64 ReLU units, one output per candidate (predicted log runtime). Weights
+ * are FP32, row-major W[out][in], laid out W1,b1,W2,b2,W3,b3 in one buffer. */
+#define MAXMULSK_MLP_IN 4
+#define MAXMULSK_MLP_HIDDEN 64
+size_t maxmulsk_mlp_num_params(void);
+
+/* Forward pass over a batch on the device (device pointers): X[batch][4] ->
+ * Y[batch][ncand]; H1, H2 (batch x 64) keep the post-ReLU activations for the
+ * backward pass (may be NULL for inference). */
+maxmulsk_status maxmulsk_mlp_forward(const float* params, const float* X, int batch,
+                                     float* H1, float* H2, float* Y, void* stream);
+/* Masked mean-squared-error loss gradient and backward pass: T[batch][ncand]
+ * targets, Mask[batch][ncand] in {0,1}; writes dParams (same layout as params,
+ * overwritten) and the loss into *loss_dev (device float, may be NULL). */
+maxmulsk_status maxmulsk_mlp_backward(const float* params, const float* X, int batch,
+                                      const float* H1, const float* H2, const float* Y,
+                                      const float* T, const float* Mask,
+                                      float* dparams, float* loss_dev, void* stream);
+/* One Adam step (device pointers): m, v moment buffers; step is 1-based. */
+maxmulsk_status maxmulsk_mlp_adam(float* params, const float* dparams, float* m, float* v,
+                                  int step, float lr, float beta1, float beta2, float eps,
+                                  void* stream);
+/* Installs host-side weights as the process-wide selector used by the planner
+ * (tier 2); NULL uninstalls it. A plan is taken from the network only where it
+ * is valid for the shape. */
+maxmulsk_status maxmulsk_selector_set(const float* host_params);
+/* Selector inference on the device through the dedicated one-wave kernel:
+ * writes the predicted log-runtime of every candidate into pred (host array of
+ * maxmulsk_num_candidates() floats) and the arg-min into *best. */
+maxmulsk_status maxmulsk_selector_predict(size_t M, size_t K, size_t N, maxmulsk_mode mode,
+                                          float* pred, maxmulsk_plan* best);
+
+/* ---- introspection --------------------------------------------------------- */
+
+/* Number of usable gfx950 devices (0 when none). */
+int maxmulsk_device_count(void);
+/* Text of the last error on this thread (never NULL). */
+const char* maxmulsk_last_error(void);
+/* Host-side view of the LDS staging layout, for known-answer tests of the
+ * swizzle: byte offset inside one pipeline stage of the 16-byte chunk that
+ * holds (row, chunk c, term) of operand op (0 = A tile, 1 = B tile), for a
+ * tile of `rows` rows (BM or BN) and depth bk, in the given mode; -1 when out
+ * of range. */
+int maxmulsk_lds_chunk_offset(int mode, int op, int rows, int bk, int row, int c, int term);
+
+#ifdef __cplusplus
+}
+#endif

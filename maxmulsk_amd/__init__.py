@@ -1,0 +1,220 @@
+"""Python mirror of the MaxMulSK C ABI on MI355X (ctypes over libmaxmulsk.so).
+
+The reference exposes two C entry points, ``maxmulsk_sgemm_v3`` and
+``maxmulsk_sgemm_v4`` (capi/maxmulsk.h:32-37): row-major FP32 ``C = A*B`` with
+dimensions in (M, K, N) order and C overwritten. ``sgemm_v3`` / ``sgemm_v4``
+below take the same arguments (numpy arrays in host memory, or torch tensors /
+raw device pointers on the GPU) and return the same status values. The
+extensions (modes, plans, workspaces, grouped launches, autotuner, learned
+selector) follow include/maxmulsk.h.
+
+The library is loaded from this directory; importing this package never falls
+back to any other implementation: when libmaxmulsk.so is missing, every call
+raises ``MaxMulSKError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Any, Sequence
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmaxmulsk.so")
+
+OK = 0
+ERR_UNSUPPORTED = 1
+ERR_ALLOC = 2
+ERR_DEVICE = 3
+STATUS_NAMES = {OK: "MAXMULSK_OK", ERR_UNSUPPORTED: "MAXMULSK_ERR_UNSUPPORTED",
+                ERR_ALLOC: "MAXMULSK_ERR_ALLOC", ERR_DEVICE: "MAXMULSK_ERR_DEVICE"}
+
+MODE_PLAIN = 0
+MODE_SPLIT3 = 1
+MODE_SPLIT6 = 2
+MODES = {"plain": MODE_PLAIN, "split3": MODE_SPLIT3, "split6": MODE_SPLIT6}
+
+MLP_IN = 4
+MLP_HIDDEN = 64
+
+
+class MaxMulSKError(RuntimeError):
+    pass
+
+
+class Plan(C.Structure):
+    _fields_ = [("tile", C.c_int), ("split_k", C.c_int)]
+
+    def __repr__(self) -> str:
+        return f"Plan(tile={self.tile}, split_k={self.split_k})"
+
+    def as_tuple(self) -> tuple[int, int]:
+        return (self.tile, self.split_k)
+
+
+class Options(C.Structure):
+    _fields_ = [("mode", C.c_int), ("plan", Plan), ("stream", C.c_void_p), ("async_", C.c_int),
+                ("workspace", C.c_void_p), ("workspace_bytes", C.c_size_t)]
+
+
+class Gemm(C.Structure):
+    _fields_ = [("A", C.c_void_p), ("B", C.c_void_p), ("C", C.c_void_p),
+                ("M", C.c_size_t), ("K", C.c_size_t), ("N", C.c_size_t)]
+
+
+_SIZE = C.c_size_t
+_P = C.c_void_p
+_SIGS: dict[str, tuple[Any, list[Any]]] = {
+    "maxmulsk_sgemm_v3": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE]),
+    "maxmulsk_sgemm_v4": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE]),
+    "maxmulsk_sgemm_ex": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.POINTER(Options)]),
+    "maxmulsk_workspace_bytes": (C.c_size_t, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan)]),
+    "maxmulsk_sgemm_ws": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), _P, _SIZE, _P]),
+    "maxmulsk_choose_plan": (C.c_int, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), C.POINTER(C.c_int)]),
+    "maxmulsk_rule_plan": (Plan, [_SIZE, _SIZE, _SIZE, C.c_int]),
+    "maxmulsk_sgemm_grouped": (C.c_int, [C.POINTER(Gemm), _SIZE, C.POINTER(Options)]),
+    "maxmulsk_grouped_create": (C.c_int, [C.POINTER(Gemm), _SIZE, C.c_int, _P, C.POINTER(_P)]),
+    "maxmulsk_grouped_launch": (C.c_int, [_P, _P]),
+    "maxmulsk_grouped_info": (C.c_int, [_P, C.POINTER(_SIZE), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "maxmulsk_grouped_destroy": (None, [_P]),
+    "maxmulsk_num_candidates": (C.c_int, []),
+    "maxmulsk_candidate": (Plan, [C.c_int]),
+    "maxmulsk_autotune": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_float), C.POINTER(Plan)]),
+    "maxmulsk_cache_put": (C.c_int, [_SIZE, _SIZE, _SIZE, C.c_int, Plan]),
+    "maxmulsk_cache_clear": (None, []),
+    "maxmulsk_cache_size": (C.c_size_t, []),
+    "maxmulsk_cache_save": (C.c_int, [C.c_char_p]),
+    "maxmulsk_cache_load": (C.c_int, [C.c_char_p]),
+    "maxmulsk_mlp_num_params": (C.c_size_t, []),
+    "maxmulsk_mlp_forward": (C.c_int, [_P, _P, C.c_int, _P, _P, _P, _P]),
+    "maxmulsk_mlp_backward": (C.c_int, [_P, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "maxmulsk_mlp_adam": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P]),
+    "maxmulsk_selector_set": (C.c_int, [_P]),
+    "maxmulsk_selector_predict": (C.c_int, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(C.c_float), C.POINTER(Plan)]),
+    "maxmulsk_device_count": (C.c_int, []),
+    "maxmulsk_last_error": (C.c_char_p, []),
+    "maxmulsk_lds_chunk_offset": (C.c_int, [C.c_int] * 7),
+}
+EXPORTED = tuple(_SIGS)
+
+_lib: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libmaxmulsk.so; raises MaxMulSKError when it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MaxMulSKError(f"{LIB_PATH} is not built (run __graft_entry__.build() "
+                                "or python -m maxmulsk_amd.build)")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def last_error() -> str:
+    return lib().maxmulsk_last_error().decode()
+
+
+def device_count() -> int:
+    return lib().maxmulsk_device_count()
+
+
+def _ptr(x: Any) -> int:
+    """Address of a numpy array, torch tensor or raw integer pointer."""
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if x.dtype.__str__() not in ("torch.float32",) and not getattr(x, "_mmsk_any_dtype", False):
+            raise MaxMulSKError(f"expected float32 tensor, got {x.dtype}")
+        if not x.is_contiguous():
+            raise MaxMulSKError("tensor must be contiguous")
+        return x.data_ptr()
+    if hasattr(x, "ctypes"):
+        import numpy as np
+        if x.dtype != np.float32 or not x.flags["C_CONTIGUOUS"]:
+            raise MaxMulSKError("array must be C-contiguous float32")
+        return x.ctypes.data
+    raise MaxMulSKError(f"cannot take the address of {type(x)}")
+
+
+def _stream_ptr(stream: Any) -> int:
+    if stream is None:
+        return 0
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream  # torch.cuda.Stream
+
+
+def sgemm_v3(A: Any, B: Any, Cm: Any, M: int, K: int, N: int) -> int:
+    """maxmulsk_sgemm_v3: C = A*B, (M, K, N) order; returns the status."""
+    return lib().maxmulsk_sgemm_v3(_ptr(A), _ptr(B), _ptr(Cm), M, K, N)
+
+
+def sgemm_v4(A: Any, B: Any, Cm: Any, M: int, K: int, N: int) -> int:
+    """maxmulsk_sgemm_v4: C = A*B, (M, K, N) order; returns the status."""
+    return lib().maxmulsk_sgemm_v4(_ptr(A), _ptr(B), _ptr(Cm), M, K, N)
+
+
+def sgemm_ex(A: Any, B: Any, Cm: Any, M: int, K: int, N: int, mode: int | str = MODE_PLAIN,
+             plan: tuple[int, int] | None = None, stream: Any = None, async_: bool = False,
+             workspace: Any = None, workspace_bytes: int = 0) -> int:
+    if isinstance(mode, str):
+        mode = MODES[mode]
+    opt = Options(mode=mode, plan=Plan(*(plan or (0, 0))), stream=_stream_ptr(stream),
+                  async_=1 if async_ else 0, workspace=_ptr(workspace) if workspace is not None else None,
+                  workspace_bytes=workspace_bytes)
+    return lib().maxmulsk_sgemm_ex(_ptr(A), _ptr(B), _ptr(Cm), M, K, N, C.byref(opt))
+
+
+def workspace_bytes(M: int, K: int, N: int, mode: int = MODE_PLAIN,
+                    plan: tuple[int, int] | None = None) -> int:
+    p = C.byref(Plan(*plan)) if plan else None
+    return lib().maxmulsk_workspace_bytes(M, K, N, mode, p)
+
+
+def sgemm_ws(A: Any, B: Any, Cm: Any, M: int, K: int, N: int, mode: int, plan: tuple[int, int] | None,
+             ws: Any, ws_bytes: int, stream: Any = None) -> int:
+    p = C.byref(Plan(*plan)) if plan else None
+    return lib().maxmulsk_sgemm_ws(_ptr(A), _ptr(B), _ptr(Cm), M, K, N, mode, p,
+                                   _ptr(ws) if ws is not None else None, ws_bytes, _stream_ptr(stream))
+
+
+def choose_plan(M: int, K: int, N: int, mode: int = MODE_PLAIN) -> tuple[tuple[int, int], int]:
+    out = Plan()
+    src = C.c_int(-1)
+    st = lib().maxmulsk_choose_plan(M, K, N, mode, C.byref(out), C.byref(src))
+    if st != OK:
+        raise MaxMulSKError(last_error())
+    return out.as_tuple(), src.value
+
+
+def rule_plan(M: int, K: int, N: int, mode: int = MODE_PLAIN) -> tuple[int, int]:
+    return lib().maxmulsk_rule_plan(M, K, N, mode).as_tuple()
+
+
+def candidates() -> list[tuple[int, int]]:
+    n = lib().maxmulsk_num_candidates()
+    return [lib().maxmulsk_candidate(i).as_tuple() for i in range(n)]
+
+
+def lds_chunk_offset(mode: int, op: int, rows: int, bk: int, row: int, c: int, term: int) -> int:
+    return lib().maxmulsk_lds_chunk_offset(mode, op, rows, bk, row, c, term)
+
+
+def check(status: int) -> None:
+    if status != OK:
+        raise MaxMulSKError(f"{STATUS_NAMES.get(status, status)}: {last_error()}")
+
+
+def make_gemms(problems: Sequence[tuple[Any, Any, Any, int, int, int]]) -> Any:
+    arr = (Gemm * len(problems))()
+    for i, (a, b, c, m, k, n) in enumerate(problems):
+        arr[i] = Gemm(_ptr(a), _ptr(b), _ptr(c), m, k, n)
+    return arr

@@ -1,0 +1,115 @@
+// One-tile-per-workgroup GEMM kernels for every (mode, tile geometry), the
+// split-K reduce, and their host launchers.
+#include <hip/hip_runtime.h>
+
+#include "gemm_tile.hpp"
+#include "kernels.hpp"
+
+namespace mmsk {
+
+template <int MODE, int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
+    using T = Tile<MODE, BM, BN, BK, WGM, WGN>;
+    __shared__ __attribute__((aligned(16))) char smem[T::SMEM];
+    const int ntiles = p.tiles_m * p.tiles_n;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int split = bid / ntiles;
+    int tm, tn;
+    grouped_tile(bid - split * ntiles, p.tiles_m, p.tiles_n, tm, tn);
+    const long long kbeg = split * p.k_len;
+    const long long kend = min(p.K, kbeg + p.k_len);
+    T t;
+    t.run(p.A, p.B, p.C + split * p.c_split_stride, p.M, p.N, p.lda, p.ldb, p.ldc,
+          (long long)tm * BM, (long long)tn * BN, kbeg, kend, p.vec_ab != 0, p.vec_c != 0, smem);
+}
+
+// Sums the split-K partials in slice order (deterministic).
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const float* __restrict__ P,
+                                                            float* __restrict__ C, long long mn,
+                                                            int splits, long long stride) {
+    const long long step = (long long)gridDim.x * 256 * 4;
+    for (long long i = ((long long)blockIdx.x * 256 + threadIdx.x) * 4; i < mn; i += step) {
+        if (i + 3 < mn && (stride & 3) == 0 && ((uintptr_t)(C + i) & 15) == 0 &&
+            ((uintptr_t)(P + i) & 15) == 0) {
+            f32x4 s = *reinterpret_cast<const f32x4*>(P + i);
+            for (int k = 1; k < splits; ++k) {
+                const f32x4 v = *reinterpret_cast<const f32x4*>(P + k * stride + i);
+                s += v;
+            }
+            *reinterpret_cast<f32x4*>(C + i) = s;
+        } else {
+            for (long long j = i; j < i + 4 && j < mn; ++j) {
+                float s = P[j];
+                for (int k = 1; k < splits; ++k) s += P[k * stride + j];
+                C[j] = s;
+            }
+        }
+    }
+}
+
+template <int MODE, int CFG>
+static hipError_t launch_one(const GemmParams& p, hipStream_t stream) {
+    constexpr TileCfg c = kTileCfgs[CFG];
+    constexpr int BK = c.bk[MODE];
+    const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
+    if (grid <= 0 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, BK, c.wgm, c.wgn>), dim3((unsigned)grid),
+                       dim3(256), 0, stream, p);
+    return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) {
+    switch (cfg) {
+        case 0: return launch_one<MODE, 0>(p, stream);
+        case 1: return launch_one<MODE, 1>(p, stream);
+        case 2: return launch_one<MODE, 2>(p, stream);
+        case 3: return launch_one<MODE, 3>(p, stream);
+        case 4: return launch_one<MODE, 4>(p, stream);
+        case 5: return launch_one<MODE, 5>(p, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_gemm(int mode, int cfg, const GemmParams& p, hipStream_t stream) {
+    switch (mode) {
+        case 0: return launch_mode<0>(cfg, p, stream);
+        case 1: return launch_mode<1>(cfg, p, stream);
+        case 2: return launch_mode<2>(cfg, p, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_splitk_reduce(const float* P, float* C, long long mn, int splits,
+                                long long stride, hipStream_t stream) {
+    long long blocks = (mn + 1023) / 1024;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, P, C, mn,
+                       splits, stride);
+    return hipGetLastError();
+}
+
+}  // namespace mmsk
+
+namespace mmsk {
+
+__global__ __launch_bounds__(256) void fill_zero_u32_kernel(unsigned int* p, size_t n) {
+    const size_t step = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += step) p[i] = 0u;
+}
+
+// Graph-safe zero fill (a kernel node instead of a memset node).
+hipError_t launch_fill_zero_u32(unsigned int* p, size_t n, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_zero_u32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero_f32(float* p, size_t n, hipStream_t stream) {
+    return launch_fill_zero_u32(reinterpret_cast<unsigned int*>(p), n, stream);
+}
+
+}  // namespace mmsk

@@ -1,0 +1,372 @@
+// Device-side GEMM tile: one workgroup (4 waves) computes one BM x BN output
+// tile over one K range. Shared by the one-tile-per-workgroup kernel and the
+// grouped persistent kernel.
+//
+// Pipeline per BK-deep k-block (two LDS stages, one barrier per block):
+//   global -> registers for block kb+1 (issued first, in flight during MFMA)
+//   MFMA over the LDS stage of block kb
+//   registers -> (split to BF16 terms) -> the other LDS stage
+//   barrier
+// Edge tiles and the last partial k-block take the masked, element-wise load
+// path (zero fill outside M, N and the K range); full tiles take dwordx4 loads.
+//
+// Counterparts in the reference: pack_B_streaming
+// (sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:127-175) and the A panel transpose
+// NB_PACK_A_KP_CHUNK (:58-117) become the register -> LDS stores below;
+// micro_kernel_1x4 (:196-327) becomes the MFMA loop; svzero_za (:554) the
+// zero-initialised accumulators; store_za / the masked edge copy (:342-405,
+// :561-588) the epilogue.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lds_layout.hpp"
+
+namespace mmsk {
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// Two FP32 values -> two BF16 values (round to nearest even), packed with the
+// first value in the low half (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t pack_bf16x2(float x0, float x1) {
+    f32x2 v = {x0, x1};
+    bf16x2 h = __builtin_convertvector(v, bf16x2);
+    return __builtin_bit_cast(uint32_t, h);
+}
+
+// Exact FP32 values of a packed BF16 pair.
+__device__ __forceinline__ void unpack_bf16x2(uint32_t w, float& f0, float& f1) {
+    f0 = __uint_as_float(w << 16);
+    f1 = __uint_as_float(w & 0xffff0000u);
+}
+
+// Splits two FP32 values into TERMS BF16 terms each: hi = RN(x), then each next
+// term rounds the exact FP32 residual of the previous ones. x - hi (and the
+// second residual) is exact in FP32 because hi agrees with x in its leading bits.
+template <int TERMS>
+__device__ __forceinline__ void split_pair(float x0, float x1, uint32_t (&out)[TERMS]) {
+    const uint32_t h = pack_bf16x2(x0, x1);
+    out[0] = h;
+    float h0, h1;
+    unpack_bf16x2(h, h0, h1);
+    const float r0 = x0 - h0, r1 = x1 - h1;
+    const uint32_t m = pack_bf16x2(r0, r1);
+    out[1] = m;
+    if constexpr (TERMS == 3) {
+        float m0, m1;
+        unpack_bf16x2(m, m0, m1);
+        out[2] = pack_bf16x2(r0 - m0, r1 - m1);
+    }
+}
+
+// Workgroup id -> logical tile id so that the workgroups each XCD receives
+// (hardware dispatches round-robin: id % 8) hold a contiguous logical range,
+// which the grouped tile order below turns into a compact block of C whose A
+// and B panels stay in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int grid) {
+    const int per = grid >> 3, extra = grid & 7;
+    const int xcd = bid & 7, local = bid >> 3;
+    return (xcd < extra ? xcd * (per + 1) : extra * (per + 1) + (xcd - extra) * per) + local;
+}
+
+// Logical tile id -> (tm, tn), walking GROUP_M tile rows at a time.
+__device__ __forceinline__ void grouped_tile(int t, int tiles_m, int tiles_n, int& tm, int& tn) {
+    constexpr int GROUP_M = 8;
+    const int per_group = GROUP_M * tiles_n;
+    const int group = t / per_group;
+    const int first_m = group * GROUP_M;
+    const int gsz = min(tiles_m - first_m, GROUP_M);
+    const int in = t - group * per_group;
+    tm = first_m + in % gsz;
+    tn = in / gsz;
+}
+
+// Term index of the A (side 0) or B (side 1) factor of cross product p.
+// split3: (lo,hi) (hi,lo) (hi,hi); split6: (lo,hi) (hi,lo) (mid,mid) (mid,hi)
+// (hi,mid) (hi,hi), i.e. smallest magnitude first.
+__host__ __device__ constexpr int product_term(int mode, int p, int side) {
+    return mode == 1 ? (side == 0 ? (p == 0 ? 1 : 0) : (p == 1 ? 1 : 0))
+                     : (side == 0 ? (p == 0 ? 2 : (p == 2 || p == 3) ? 1 : 0)
+                                  : (p == 1 ? 2 : (p == 2 || p == 4) ? 1 : 0));
+}
+
+template <int MODE, int BM, int BN, int BK, int WGM, int WGN>
+struct Tile {
+    static constexpr int LU = mode_lu(MODE);
+    static constexpr int TERMS = mode_terms(MODE);
+    static constexpr int CPR = BK / LU;             // 16-byte chunks per row of a k-block
+    static constexpr int KG = CPR / 2;              // 2-chunk k-groups (one MFMA fragment deep)
+    static constexpr int A_UNITS = BM * CPR;        // A unit: one chunk of one row
+    static constexpr int B_UNITS = CPR * (BN / 4);  // B unit: LU k-rows x 4 columns
+    static constexpr int UA = (A_UNITS + 255) / 256;
+    static constexpr int UB = (B_UNITS + 255) / 256;
+    static constexpr int A_BYTES = operand_bytes(MODE, BM, BK);
+    static constexpr int B_BYTES = operand_bytes(MODE, BN, BK);
+    static constexpr int STAGE = A_BYTES + B_BYTES;
+    static constexpr int SMEM = 2 * STAGE;
+    static constexpr int WTM = BM / WGM, WTN = BN / WGN;
+    static constexpr int TM = WTM / 32, TN = WTN / 32;
+
+    static_assert(WGM * WGN == 4, "four waves per workgroup");
+    static_assert(BK % (2 * LU) == 0, "k-block must hold whole fragments");
+    static_assert(TM >= 1 && TN >= 1, "wave tile must hold a 32x32 block");
+    static_assert(SMEM <= 160 * 1024, "LDS budget of one CU");
+
+    f32x4 ra[UA][LU / 4];  // A unit i: LU consecutive k of one row
+    f32x4 rb[UB][LU];      // B unit i: row e holds 4 consecutive columns
+    f32x16 acc[TM][TN];
+
+    __device__ __forceinline__ void load(const float* __restrict__ A, const float* __restrict__ B,
+                                         long long M, long long N, long long lda, long long ldb,
+                                         long long m0, long long n0, long long k0, long long kend,
+                                         bool fast) {
+        const int tid = threadIdx.x;
+        if (fast) {
+#pragma unroll
+            for (int i = 0; i < UA; ++i) {
+                const int u = tid + 256 * i;
+                if (A_UNITS % 256 == 0 || u < A_UNITS) {
+                    const int row = u / CPR, c = u % CPR;
+                    const float* src = A + (m0 + row) * lda + k0 + c * LU;
+#pragma unroll
+                    for (int q = 0; q < LU / 4; ++q)
+                        ra[i][q] = *reinterpret_cast<const f32x4*>(src + 4 * q);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < UB; ++i) {
+                const int u = tid + 256 * i;
+                if (B_UNITS % 256 == 0 || u < B_UNITS) {
+                    const int n4 = u % (BN / 4), c = u / (BN / 4);
+                    const float* src = B + (k0 + c * LU) * ldb + n0 + 4 * n4;
+#pragma unroll
+                    for (int e = 0; e < LU; ++e)
+                        rb[i][e] = *reinterpret_cast<const f32x4*>(src + e * ldb);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < UA; ++i) {
+                const int u = tid + 256 * i;
+                if (A_UNITS % 256 == 0 || u < A_UNITS) {
+                    const int row = u / CPR, c = u % CPR;
+                    const long long gm = m0 + row, gk = k0 + c * LU;
+#pragma unroll
+                    for (int e = 0; e < LU; ++e) {
+                        float x = 0.f;
+                        if (gm < M && gk + e < kend) x = A[gm * lda + gk + e];
+                        ra[i][e / 4][e % 4] = x;
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < UB; ++i) {
+                const int u = tid + 256 * i;
+                if (B_UNITS % 256 == 0 || u < B_UNITS) {
+                    const int n4 = u % (BN / 4), c = u / (BN / 4);
+                    const long long gn = n0 + 4 * n4;
+#pragma unroll
+                    for (int e = 0; e < LU; ++e) {
+                        const long long gk = k0 + c * LU + e;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            float x = 0.f;
+                            if (gk < kend && gn + j < N) x = B[gk * ldb + gn + j];
+                            rb[i][e][j] = x;
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    __device__ __forceinline__ void store(char* stage) const {
+        const int tid = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < UA; ++i) {
+            const int u = tid + 256 * i;
+            if (A_UNITS % 256 == 0 || u < A_UNITS) {
+                const int row = u / CPR, c = u % CPR;
+                if constexpr (MODE == 0) {
+                    *reinterpret_cast<f32x4*>(stage + chunk_offset(MODE, 0, BM, BK, row, c, 0)) = ra[i][0];
+                } else {
+                    uint32_t w[TERMS][4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        uint32_t t[TERMS];
+                        split_pair<TERMS>(ra[i][q / 2][2 * (q % 2)], ra[i][q / 2][2 * (q % 2) + 1], t);
+#pragma unroll
+                        for (int s = 0; s < TERMS; ++s) w[s][q] = t[s];
+                    }
+#pragma unroll
+                    for (int s = 0; s < TERMS; ++s) {
+                        const u32x4 v = {w[s][0], w[s][1], w[s][2], w[s][3]};
+                        *reinterpret_cast<u32x4*>(stage + chunk_offset(MODE, 0, BM, BK, row, c, s)) = v;
+                    }
+                }
+            }
+        }
+        char* bstage = stage + A_BYTES;
+#pragma unroll
+        for (int i = 0; i < UB; ++i) {
+            const int u = tid + 256 * i;
+            if (B_UNITS % 256 == 0 || u < B_UNITS) {
+                const int n4 = u % (BN / 4), c = u / (BN / 4);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int n = 4 * n4 + j;
+                    if constexpr (MODE == 0) {
+                        const f32x4 v = {rb[i][0][j], rb[i][1][j], rb[i][2][j], rb[i][3][j]};
+                        *reinterpret_cast<f32x4*>(bstage + chunk_offset(MODE, 1, BN, BK, n, c, 0)) = v;
+                    } else {
+                        uint32_t w[TERMS][4];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            uint32_t t[TERMS];
+                            split_pair<TERMS>(rb[i][2 * q][j], rb[i][2 * q + 1][j], t);
+#pragma unroll
+                            for (int s = 0; s < TERMS; ++s) w[s][q] = t[s];
+                        }
+#pragma unroll
+                        for (int s = 0; s < TERMS; ++s) {
+                            const u32x4 v = {w[s][0], w[s][1], w[s][2], w[s][3]};
+                            *reinterpret_cast<u32x4*>(bstage + chunk_offset(MODE, 1, BN, BK, n, c, s)) = v;
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    __device__ __forceinline__ void compute(const char* stage) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int wm = wave / WGN, wn = wave % WGN;
+        const int r = lane & 31, h = lane >> 5;
+        const char* bstage = stage + A_BYTES;
+#pragma unroll
+        for (int kg = 0; kg < KG; ++kg) {
+            const int c = 2 * kg + h;
+            if constexpr (MODE == 0) {
+                f32x4 a[TM], b[TN];
+#pragma unroll
+                for (int ti = 0; ti < TM; ++ti)
+                    a[ti] = *reinterpret_cast<const f32x4*>(
+                        stage + chunk_offset(MODE, 0, BM, BK, wm * WTM + ti * 32 + r, c, 0));
+#pragma unroll
+                for (int tj = 0; tj < TN; ++tj)
+                    b[tj] = *reinterpret_cast<const f32x4*>(
+                        bstage + chunk_offset(MODE, 1, BN, BK, wn * WTN + tj * 32 + r, c, 0));
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+                        for (int tj = 0; tj < TN; ++tj)
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                                b[tj][s], a[ti][s], acc[ti][tj], 0, 0, 0);
+            } else {
+                bf16x8 a[TERMS][TM], b[TERMS][TN];
+#pragma unroll
+                for (int s = 0; s < TERMS; ++s) {
+#pragma unroll
+                    for (int ti = 0; ti < TM; ++ti)
+                        a[s][ti] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                            stage + chunk_offset(MODE, 0, BM, BK, wm * WTM + ti * 32 + r, c, s)));
+#pragma unroll
+                    for (int tj = 0; tj < TN; ++tj)
+                        b[s][tj] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                            bstage + chunk_offset(MODE, 1, BN, BK, wn * WTN + tj * 32 + r, c, s)));
+                }
+                // Cross products, smallest first, all into the one FP32
+                // accumulator: split3 = lo*hi + hi*lo + hi*hi; split6 adds the
+                // mid terms (Henry, Tang & Heinecke, ARITH 2019).
+                constexpr int NP = MODE == 1 ? 3 : 6;
+#pragma unroll
+                for (int p = 0; p < NP; ++p) {
+                    const int ta = product_term(MODE, p, 0);
+                    const int tb = product_term(MODE, p, 1);
+#pragma unroll
+                    for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+                        for (int tj = 0; tj < TN; ++tj)
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                b[tb][tj], a[ta][ti], acc[ti][tj], 0, 0, 0);
+                }
+            }
+        }
+    }
+
+    // Accumulator (lane l, register 4q + e) of block (ti, tj) holds
+    // C[m0 + wm*WTM + ti*32 + l%32][n0 + wn*WTN + tj*32 + 8q + 4(l/32) + e]:
+    // four consecutive columns per q, stored as one dwordx4 where aligned.
+    __device__ __forceinline__ void epilogue(float* __restrict__ C, long long M, long long N,
+                                             long long ldc, long long m0, long long n0,
+                                             bool vec_c) const {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int wm = wave / WGN, wn = wave % WGN;
+        const int r = lane & 31, h = lane >> 5;
+#pragma unroll
+        for (int ti = 0; ti < TM; ++ti) {
+            const long long m = m0 + wm * WTM + ti * 32 + r;
+            if (m >= M) continue;
+            float* crow = C + m * ldc;
+#pragma unroll
+            for (int tj = 0; tj < TN; ++tj) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const long long n = n0 + wn * WTN + tj * 32 + 8 * q + 4 * h;
+                    const f32x4 v = {acc[ti][tj][4 * q + 0], acc[ti][tj][4 * q + 1],
+                                     acc[ti][tj][4 * q + 2], acc[ti][tj][4 * q + 3]};
+                    if (vec_c && n + 3 < N) {
+                        *reinterpret_cast<f32x4*>(crow + n) = v;
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (n + e < N) crow[n + e] = v[e];
+                    }
+                }
+            }
+        }
+    }
+
+    // Whole tile: C[m0:m0+BM, n0:n0+BN] = A[m0:, kbeg:kend] * B[kbeg:kend, n0:].
+    // smem holds SMEM bytes; on return every wave has passed the last barrier,
+    // so the caller may reuse smem for the next tile without another barrier.
+    __device__ __forceinline__ void run(const float* __restrict__ A, const float* __restrict__ B,
+                                        float* __restrict__ C, long long M, long long N,
+                                        long long lda, long long ldb, long long ldc,
+                                        long long m0, long long n0, long long kbeg, long long kend,
+                                        bool vec_ab, bool vec_c, char* smem) {
+#pragma unroll
+        for (int ti = 0; ti < TM; ++ti)
+#pragma unroll
+            for (int tj = 0; tj < TN; ++tj)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[ti][tj][e] = 0.f;
+
+        const bool full_mn = (m0 + BM <= M) && (n0 + BN <= N) && vec_ab;
+        const long long nkb = (kend - kbeg + BK - 1) / BK;
+        load(A, B, M, N, lda, ldb, m0, n0, kbeg, kend, full_mn && kbeg + BK <= kend);
+        store(smem);
+        __syncthreads();
+        for (long long kb = 0; kb < nkb; ++kb) {
+            char* cur = smem + (kb & 1) * STAGE;
+            char* nxt = smem + ((kb + 1) & 1) * STAGE;
+            const bool more = kb + 1 < nkb;
+            const long long k1 = kbeg + (kb + 1) * BK;
+            if (more) load(A, B, M, N, lda, ldb, m0, n0, k1, kend, full_mn && k1 + BK <= kend);
+            compute(cur);
+            if (more) store(nxt);
+            __syncthreads();
+        }
+        epilogue(C, M, N, ldc, m0, n0, vec_c);
+    }
+};
+
+}  // namespace mmsk

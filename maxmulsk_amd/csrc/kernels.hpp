@@ -1,0 +1,104 @@
+// Host-visible kernel launch interface (no HIP device code here).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace mmsk {
+
+// Tile geometries compiled for every mode. Four waves (256 threads) per
+// workgroup, laid out wgm x wgn over the output tile.
+struct TileCfg {
+    int bm, bn;
+    int wgm, wgn;
+    int bk[3];  // tile depth per mode: plain, split3, split6
+};
+
+constexpr int kNumTileCfgs = 6;
+constexpr TileCfg kTileCfgs[kNumTileCfgs] = {
+    {64, 64, 2, 2, {32, 32, 32}},
+    {64, 128, 2, 2, {32, 32, 32}},
+    {64, 256, 1, 4, {32, 32, 32}},
+    {128, 128, 2, 2, {32, 32, 32}},
+    {128, 256, 2, 2, {32, 32, 32}},
+    {256, 256, 2, 2, {32, 32, 16}},
+};
+
+// One launch of the tiled kernel: an output grid of tiles_m x tiles_n tiles,
+// each computed over `splits` K slices of k_len elements. With splits > 1 every
+// slice writes its own FP32 partial matrix at C + split * c_split_stride and a
+// reduce kernel sums them in slice order.
+struct GemmParams {
+    const float* A;
+    const float* B;
+    float* C;
+    long long M, N, K;
+    long long lda, ldb, ldc;
+    long long k_len;
+    long long c_split_stride;
+    int splits;
+    int tiles_m, tiles_n;
+    int vec_ab;  // A and B rows 16-byte aligned (K % 4 == 0, N % 4 == 0, aligned bases)
+    int vec_c;   // C rows 16-byte aligned
+};
+
+hipError_t launch_gemm(int mode, int cfg, const GemmParams& p, hipStream_t stream);
+
+// C[i] = sum_{s < splits} P[s * stride + i] for i < M*N, summed in s order.
+hipError_t launch_splitk_reduce(const float* P, float* C, long long mn, int splits,
+                                long long stride, hipStream_t stream);
+
+// ---- grouped persistent launch ------------------------------------------------
+
+// One problem of a grouped launch as the device sees it.
+struct GroupProblem {
+    const float* A;
+    const float* B;
+    float* C;      // final output
+    float* P;      // split-K partials (nullptr when splits == 1)
+    long long M, N, K;
+    long long k_len;
+    int splits;
+    int cfg;       // tile geometry index
+    int tiles_m, tiles_n;
+    int vec_ab, vec_c;
+    int item_begin;   // first work item of this problem in the flattened list
+    int reduce_begin; // first reduce block of this problem (split problems only)
+};
+
+// Persistent kernel over the flattened (problem, tile, k-slice) work list;
+// work items are claimed from an atomic counter that starts at 0 (*counter must
+// be zero on entry; the kernel leaves it at total + grid). Each geometry gets its
+// own persistent kernel; `cfg` selects which problems it serves via the item
+// ranges in items_of_cfg.
+struct GroupLaunch {
+    const GroupProblem* problems;
+    int num_problems;
+    const int* item_problem;  // work item -> problem index (items sorted by problem)
+    int total_items;
+    unsigned int* counter;
+};
+hipError_t launch_grouped(int mode, int cfg, const GroupLaunch& g, int grid, hipStream_t stream);
+// Reduce for every split problem: reduce block -> problem via block_problem.
+hipError_t launch_grouped_reduce(const GroupProblem* problems, const int* block_problem,
+                                 int total_blocks, hipStream_t stream);
+// Zero-fill (graph-safe replacement for hipMemsetAsync).
+hipError_t launch_fill_zero_u32(unsigned int* p, size_t n, hipStream_t stream);
+hipError_t launch_fill_zero_f32(float* p, size_t n, hipStream_t stream);
+
+// ---- MLP (learned selector) ---------------------------------------------------
+constexpr int kMlpIn = 4;
+constexpr int kMlpHidden = 64;
+hipError_t launch_mlp_forward(const float* params, const float* X, int batch, int nout,
+                              float* H1, float* H2, float* Y, hipStream_t stream);
+hipError_t launch_mlp_backward(const float* params, const float* X, int batch, int nout,
+                               const float* H1, const float* H2, const float* Y,
+                               const float* T, const float* Mask, float* dparams,
+                               float* loss, hipStream_t stream);
+hipError_t launch_mlp_adam(float* params, const float* g, float* m, float* v, int n, int step,
+                           float lr, float b1, float b2, float eps, hipStream_t stream);
+hipError_t launch_mlp_infer(const float* params, const float* x, int nout, float* y,
+                            hipStream_t stream);
+
+}  // namespace mmsk

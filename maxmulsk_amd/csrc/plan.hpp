@@ -1,0 +1,69 @@
+// Launch planning: plan validation, tile geometry, split-K resolution and the
+// plan tiers (environment override -> autotuner cache -> learned selector ->
+// shape rule). Pure host C++; no HIP calls except the selector tier.
+#pragma once
+
+#include <cstddef>
+
+#include "maxmulsk.h"
+
+namespace mmsk {
+
+constexpr int kCandTiles[3] = {64, 128, 256};
+constexpr int kCandSplits[5] = {1, 2, 4, 8, 16};
+constexpr int kNumCandidates = 15;  // index = tile_index * 5 + split_index
+constexpr int kMaxSplit = 16;
+constexpr int kNumCUs = 256;        // MI355X compute units
+
+// Candidate i of the fixed candidate set.
+maxmulsk_plan candidate(int i);
+// Index of a plan in the candidate set, -1 when it is not a candidate.
+int candidate_index(maxmulsk_plan p);
+
+// Tile geometry index (into kTileCfgs) of tile edge `tile` for M rows: the N
+// edge is the tile edge; the M edge shrinks to 64 or 128 for narrow M.
+int cfg_for(int tile, size_t M);
+
+// A plan resolved against a shape: geometry, K slicing, grid and workspace.
+struct Resolved {
+    int cfg;
+    int bm, bn, bk;
+    int splits;          // K slices actually used (<= plan.split_k)
+    long long k_len;     // K elements per slice, a multiple of bk
+    int tiles_m, tiles_n;
+    size_t ws_bytes;     // split-K partials (0 when splits == 1)
+};
+
+// Counterpart of classify (sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:410-423)
+// and the clamps of run_multiplication (:638-654): rejects a plan whose tile is
+// not 64/128/256 or whose split is outside 1..16, then clamps the split to the
+// shape (no empty slices). Shapes must be non-empty.
+bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* out);
+
+// Shape rule (tier 3): the largest tile that still gives at least half a wave
+// of workgroups, then split-K until the grid covers the CUs, keeping slices at
+// least 256 deep.
+maxmulsk_plan rule_plan(size_t M, size_t K, size_t N, int mode);
+
+// MAXMULSK_PLAN="tile,split" (the analogue of MAXMULSK_BLOCKING,
+// sme/support/gemm_tuning.cpp:66-77). Malformed values are ignored.
+bool env_plan(maxmulsk_plan* out);
+// MAXMULSK_MODE=plain|split3|split6 for the two reference entry points;
+// anything else leaves `dflt`.
+int env_mode(int dflt);
+
+// Autotuner cache.
+bool cache_get(size_t M, size_t K, size_t N, int mode, maxmulsk_plan* out);
+void cache_put(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p);
+void cache_clear();
+size_t cache_size();
+bool cache_save(const char* path, const char* device_name);
+int cache_load(const char* path, const char* device_name);  // 0 ok, 1 unsupported, 2 io
+
+// Learned selector tier (defined with the MLP code).
+bool selector_pick(size_t M, size_t K, size_t N, int mode, maxmulsk_plan* out);
+
+// Full planner: tiers 0..3. source receives the tier used.
+maxmulsk_plan choose_plan(size_t M, size_t K, size_t N, int mode, int* source);
+
+}  // namespace mmsk

@@ -1,0 +1,286 @@
+#include "runtime.hpp"
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace mmsk {
+
+namespace {
+thread_local std::string t_err;
+constexpr int kMaxDevices = 64;
+DevicePools g_pools[kMaxDevices];
+std::once_flag g_dev_once;
+int g_dev_count = 0;
+bool g_dev_ok[kMaxDevices] = {};
+
+void discover() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        n = 0;
+    }
+    if (n > kMaxDevices) n = kMaxDevices;
+    int ok = 0;
+    for (int d = 0; d < n; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) != hipSuccess) {
+            (void)hipGetLastError();
+            continue;
+        }
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) {
+            g_dev_ok[d] = true;
+            ++ok;
+        }
+    }
+    g_dev_count = ok;
+}
+}  // namespace
+
+maxmulsk_status fail(maxmulsk_status st, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_err = buf;
+    return st;
+}
+
+const char* last_error() { return t_err.c_str(); }
+
+int usable_devices() {
+    std::call_once(g_dev_once, discover);
+    return g_dev_count;
+}
+
+bool device_usable(int dev) {
+    usable_devices();
+    return dev >= 0 && dev < kMaxDevices && g_dev_ok[dev];
+}
+
+std::string device_name(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return "unknown";
+    }
+    std::string arch(prop.gcnArchName);
+    const size_t colon = arch.find(':');
+    if (colon != std::string::npos) arch.resize(colon);
+    return arch + ":" + prop.name;
+}
+
+bool is_device_ptr(const void* p, int* dev) {
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof attr);
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) {
+        *dev = attr.device;
+        return true;
+    }
+    return false;
+}
+
+DeviceGuard::DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) {
+        (void)hipGetLastError();
+        prev = -1;
+    }
+    if (prev != dev) (void)hipSetDevice(dev);
+}
+
+DeviceGuard::~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+}
+
+DevicePools& pools(int dev) { return g_pools[dev]; }
+
+bool grow(void** buf, size_t* cap, size_t bytes) {
+    if (*cap >= bytes && *buf) return true;
+    const size_t want = (bytes + (2u << 20) - 1) / (2u << 20) * (2u << 20);
+    if (*buf) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(*buf);
+        *buf = nullptr;
+        *cap = 0;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    *buf = p;
+    *cap = want;
+    return true;
+}
+
+hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, size_t K, size_t N,
+                           int mode, const Resolved& r, void* ws, hipStream_t stream) {
+    GemmParams p{};
+    p.A = A;
+    p.B = B;
+    p.M = (long long)M;
+    p.N = (long long)N;
+    p.K = (long long)K;
+    p.lda = (long long)K;
+    p.ldb = (long long)N;
+    p.ldc = (long long)N;
+    p.k_len = r.k_len;
+    p.splits = r.splits;
+    p.tiles_m = r.tiles_m;
+    p.tiles_n = r.tiles_n;
+    p.vec_ab = (K % 4 == 0 && N % 4 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0);
+    if (r.splits > 1) {
+        p.C = static_cast<float*>(ws);
+        p.c_split_stride = (long long)(M * N);
+        p.vec_c = (N % 4 == 0);
+    } else {
+        p.C = C;
+        p.c_split_stride = 0;
+        p.vec_c = (N % 4 == 0 && ((uintptr_t)C & 15) == 0);
+    }
+    hipError_t e = launch_gemm(mode, r.cfg, p, stream);
+    if (e != hipSuccess || r.splits == 1) return e;
+    return launch_splitk_reduce(static_cast<const float*>(ws), C, (long long)(M * N), r.splits,
+                                (long long)(M * N), stream);
+}
+
+maxmulsk_status run_gemm(const GemmCall& c) {
+    // Empty-shape contract of capi/maxmulsk.cpp:13-20: nothing to do when M or N
+    // is 0; with K == 0 the product is the zero matrix.
+    if (c.M == 0 || c.N == 0) return MAXMULSK_OK;
+    if (!c.C) return fail(MAXMULSK_ERR_UNSUPPORTED, "C is null");
+    if (c.mode < 0 || c.mode > 2) return fail(MAXMULSK_ERR_UNSUPPORTED, "unknown mode %d", c.mode);
+    if (c.M > 0x7fffffffULL || c.N > 0x7fffffffULL || c.K > 0x7fffffffULL ||
+        c.M * c.N > (size_t)1 << 40)
+        return fail(MAXMULSK_ERR_UNSUPPORTED, "shape too large");
+    int cdev = -1;
+    const bool c_dev = usable_devices() > 0 && is_device_ptr(c.C, &cdev);
+    if (c.K == 0) {
+        if (!c_dev) {
+            std::memset(c.C, 0, c.M * c.N * sizeof(float));
+            return MAXMULSK_OK;
+        }
+        DeviceGuard g(cdev);
+        if (launch_fill_zero_f32(c.C, c.M * c.N, c.stream) != hipSuccess ||
+            (!c.async && hipStreamSynchronize(c.stream) != hipSuccess)) {
+            (void)hipGetLastError();
+            return fail(MAXMULSK_ERR_DEVICE, "zero fill of C failed");
+        }
+        return MAXMULSK_OK;
+    }
+    if (!c.A || !c.B) return fail(MAXMULSK_ERR_UNSUPPORTED, "A or B is null");
+    // An explicit plan is validated before anything else (classify runs before
+    // any allocation in the reference, ncblock.cpp:627-636).
+    Resolved r;
+    if (c.have_plan && !resolve(c.M, c.K, c.N, c.mode, c.plan, &r))
+        return fail(MAXMULSK_ERR_UNSUPPORTED, "plan {%d,%d} rejected for %zux%zux%zu", c.plan.tile,
+                    c.plan.split_k, c.M, c.K, c.N);
+    if (usable_devices() == 0) return fail(MAXMULSK_ERR_UNSUPPORTED, "no gfx950 device");
+
+    int adev = -1, bdev = -1;
+    const bool a_dev = is_device_ptr(c.A, &adev);
+    const bool b_dev = is_device_ptr(c.B, &bdev);
+    int dev = c_dev ? cdev : (a_dev ? adev : (b_dev ? bdev : -1));
+    if (dev < 0 && hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MAXMULSK_ERR_DEVICE, "hipGetDevice failed");
+    }
+    if ((a_dev && adev != dev) || (b_dev && bdev != dev))
+        return fail(MAXMULSK_ERR_UNSUPPORTED, "A, B and C live on different devices");
+    if (!device_usable(dev)) return fail(MAXMULSK_ERR_UNSUPPORTED, "device %d is not gfx950", dev);
+    const bool staged = !(a_dev && b_dev && c_dev);
+    if (staged && c.async) return fail(MAXMULSK_ERR_UNSUPPORTED, "async calls need device pointers");
+    if (staged && c.ws_required) return fail(MAXMULSK_ERR_UNSUPPORTED, "the workspace entry point needs device pointers");
+
+    maxmulsk_plan plan = c.plan;
+    if (!c.have_plan) {
+        if (c.rule_only) {
+            if (!env_plan(&plan)) plan = rule_plan(c.M, c.K, c.N, c.mode);
+        } else {
+            plan = choose_plan(c.M, c.K, c.N, c.mode, nullptr);
+        }
+    }
+    if (!resolve(c.M, c.K, c.N, c.mode, plan, &r))
+        return fail(MAXMULSK_ERR_UNSUPPORTED, "plan {%d,%d} rejected for %zux%zux%zu", plan.tile,
+                    plan.split_k, c.M, c.K, c.N);
+
+    if (c.ws_required || c.ws) {
+        if (r.ws_bytes > 0 && (!c.ws || c.ws_bytes < r.ws_bytes || ((uintptr_t)c.ws & 255) != 0))
+            return fail(MAXMULSK_ERR_ALLOC, "workspace null, short (%zu < %zu) or misaligned",
+                        c.ws_bytes, r.ws_bytes);
+    }
+
+    DeviceGuard guard(dev);
+    DevicePools& pl = pools(dev);
+    std::unique_lock<std::mutex> lock(pl.mu);
+
+    const float* A = c.A;
+    const float* B = c.B;
+    float* C = c.C;
+    if (staged) {
+        const size_t a_bytes = c.M * c.K * sizeof(float), b_bytes = c.K * c.N * sizeof(float),
+                     c_bytes = c.M * c.N * sizeof(float);
+        auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+        const size_t need = (a_dev ? 0 : up(a_bytes)) + (b_dev ? 0 : up(b_bytes)) + (c_dev ? 0 : up(c_bytes));
+        if (!grow(&pl.stage, &pl.stage_cap, need)) return fail(MAXMULSK_ERR_ALLOC, "staging allocation of %zu bytes failed", need);
+        char* s = static_cast<char*>(pl.stage);
+        if (!a_dev) {
+            if (hipMemcpyAsync(s, c.A, a_bytes, hipMemcpyHostToDevice, c.stream) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(MAXMULSK_ERR_DEVICE, "H2D copy of A failed");
+            }
+            A = reinterpret_cast<const float*>(s);
+            s += up(a_bytes);
+        }
+        if (!b_dev) {
+            if (hipMemcpyAsync(s, c.B, b_bytes, hipMemcpyHostToDevice, c.stream) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(MAXMULSK_ERR_DEVICE, "H2D copy of B failed");
+            }
+            B = reinterpret_cast<const float*>(s);
+            s += up(b_bytes);
+        }
+        if (!c_dev) C = reinterpret_cast<float*>(s);
+    }
+
+    void* ws = c.ws;
+    const bool pooled = r.ws_bytes > 0 && !ws;
+    if (pooled) {
+        if (!grow(&pl.ws, &pl.ws_cap, r.ws_bytes)) return fail(MAXMULSK_ERR_ALLOC, "workspace allocation of %zu bytes failed", r.ws_bytes);
+        if (!pl.ws_event && hipEventCreateWithFlags(&pl.ws_event, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(MAXMULSK_ERR_DEVICE, "event creation failed");
+        }
+        if (pl.ws_event_recorded && hipStreamWaitEvent(c.stream, pl.ws_event, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(MAXMULSK_ERR_DEVICE, "stream wait failed");
+        }
+        ws = pl.ws;
+    }
+
+    hipError_t e = launch_resolved(A, B, C, c.M, c.K, c.N, c.mode, r, ws, c.stream);
+    if (e == hipSuccess && pooled) {
+        e = hipEventRecord(pl.ws_event, c.stream);
+        pl.ws_event_recorded = (e == hipSuccess);
+    }
+    if (e == hipSuccess && staged && !c_dev)
+        e = hipMemcpyAsync(c.C, C, c.M * c.N * sizeof(float), hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess && !c.async) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MAXMULSK_ERR_DEVICE, "GEMM launch failed: %s", hipGetErrorString(e));
+    }
+    return MAXMULSK_OK;
+}
+
+}  // namespace mmsk

@@ -1,0 +1,79 @@
+// Host runtime shared by the C-ABI entry points: device discovery, per-device
+// workspace and staging pools, pointer classification, error text.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <mutex>
+#include <string>
+
+#include "maxmulsk.h"
+#include "plan.hpp"
+
+namespace mmsk {
+
+// Sets the thread's last-error text (printf style) and returns `st`.
+maxmulsk_status fail(maxmulsk_status st, const char* fmt, ...);
+const char* last_error();
+
+// Number of visible gfx950 devices (0 when there is no GPU or HIP fails).
+int usable_devices();
+bool device_usable(int dev);
+// "gfx950:<marketing name>" of a device, for cache files.
+std::string device_name(int dev);
+
+// True when p is device (or managed) memory; *dev receives its device.
+bool is_device_ptr(const void* p, int* dev);
+
+// Makes `dev` current for the lifetime of the guard.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev);
+    ~DeviceGuard();
+};
+
+// Per-device pools. The workspace pool serves split-K partials of calls that
+// bring no workspace of their own; every use is stream-ordered behind the
+// previous use through `ws_event`, so overlapping asynchronous calls on
+// different streams never share partials. The staging pool holds device copies
+// of host operands; it is only used by synchronous calls under `mu`.
+struct DevicePools {
+    std::mutex mu;
+    void* ws = nullptr;
+    size_t ws_cap = 0;
+    hipEvent_t ws_event = nullptr;
+    bool ws_event_recorded = false;
+    void* stage = nullptr;
+    size_t stage_cap = 0;
+};
+DevicePools& pools(int dev);
+
+// Grows a pool buffer to at least `bytes` (rounded up to 2 MiB). Waits for the
+// device before freeing the old buffer. Returns false on allocation failure.
+bool grow(void** buf, size_t* cap, size_t bytes);
+
+// The GEMM launch itself (no contract checks): all pointers are device
+// pointers, `ws` holds r.ws_bytes when r.splits > 1.
+hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, size_t K, size_t N,
+                           int mode, const Resolved& r, void* ws, hipStream_t stream);
+
+// Full entry point behind every GEMM symbol.
+struct GemmCall {
+    const float* A;
+    const float* B;
+    float* C;
+    size_t M, K, N;
+    int mode;
+    bool have_plan;       // use `plan` instead of the planner
+    maxmulsk_plan plan;
+    bool rule_only;       // planner restricted to override + shape rule (v3)
+    hipStream_t stream;
+    bool async;
+    void* ws;             // caller workspace (nullptr: pool)
+    size_t ws_bytes;
+    bool ws_required;     // caller promised a workspace (the _ws entry point)
+};
+maxmulsk_status run_gemm(const GemmCall& c);
+
+}  // namespace mmsk

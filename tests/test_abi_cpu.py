@@ -1,0 +1,273 @@
+"""C-ABI library on the CPU: loads, exports every declared symbol, honours the
+host-side contract (empty shapes, no device, plan validation, planner tiers,
+cache file), and the LDS staging layout passes its known-answer and
+bank-conflict checks."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import maxmulsk_amd as mm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "maxmulsk.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(maxmulsk_\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = mm.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(mm.EXPORTED)
+
+
+def test_status_values_match_reference_enum():
+    # capi/maxmulsk.h:24-28
+    assert (mm.OK, mm.ERR_UNSUPPORTED, mm.ERR_ALLOC) == (0, 1, 2)
+
+
+def test_empty_shapes_contract():
+    # capi/maxmulsk.cpp:13-20: M or N == 0 -> OK, nothing touched
+    C = np.full((3, 5), 7.0, np.float32)
+    A = np.ones((3, 4), np.float32)
+    B = np.ones((4, 5), np.float32)
+    for fn in (mm.sgemm_v3, mm.sgemm_v4):
+        C[:] = 7.0
+        assert fn(A, B, C, 0, 4, 5) == mm.OK
+        assert fn(A, B, C, 3, 4, 0) == mm.OK
+        assert np.all(C == 7.0)
+        # K == 0 -> C cleared
+        C[:] = 7.0
+        assert fn(A, B, C, 3, 0, 5) == mm.OK
+        assert np.all(C == 0.0)
+    # K == 0 with null A/B is fine: they are never read
+    C[:] = 7.0
+    assert mm.sgemm_v4(None, None, C, 3, 0, 5) == mm.OK and np.all(C == 0.0)
+
+
+@pytest.mark.skipif(os.environ.get("MAXMULSK_EXPECT_GPU") == "1", reason="GPU present")
+def test_no_device_is_an_error_not_a_crash():
+    if mm.device_count() > 0:
+        pytest.skip("a gfx950 device is present")
+    A = np.ones((8, 8), np.float32)
+    C = np.full((8, 8), 3.0, np.float32)
+    for fn in (mm.sgemm_v3, mm.sgemm_v4):
+        assert fn(A, A, C, 8, 8, 8) == mm.ERR_UNSUPPORTED
+        assert np.all(C == 3.0)
+        assert "gfx950" in mm.last_error()
+
+
+def test_null_pointers_rejected():
+    C = np.zeros((2, 2), np.float32)
+    assert mm.sgemm_v4(None, None, C, 2, 2, 2) == mm.ERR_UNSUPPORTED
+    assert mm.sgemm_v4(None, None, None, 2, 2, 2) == mm.ERR_UNSUPPORTED
+
+
+def test_invalid_plan_rejected_before_anything_runs():
+    A = np.ones((8, 8), np.float32)
+    C = np.full((8, 8), 9.0, np.float32)
+    for plan in [(32, 1), (100, 1), (64, 0), (64, 17), (256, -1)]:
+        assert mm.sgemm_ex(A, A, C, 8, 8, 8, plan=plan) == mm.ERR_UNSUPPORTED
+        assert np.all(C == 9.0)
+        assert mm.workspace_bytes(8, 8, 8, 0, plan) == 0
+    assert mm.sgemm_ex(A, A, C, 8, 8, 8, mode=7) == mm.ERR_UNSUPPORTED
+
+
+def test_candidates_and_rule_plan():
+    cands = mm.candidates()
+    assert len(cands) == 15
+    assert cands == [(t, s) for t in (64, 128, 256) for s in (1, 2, 4, 8, 16)]
+    assert mm.rule_plan(4096, 4096, 4096) == (256, 1)
+    # narrow M: small tiles, K split to fill the CUs
+    t, s = mm.rule_plan(64, 8192, 512)
+    assert t == 64 and s == 16
+    t, s = mm.rule_plan(64, 512, 32768)
+    assert s >= 1 and t in (64, 128, 256)
+    for shape in [(1, 1, 1), (1023, 999, 1025), (67, 8191, 513), (4096, 1536, 24576)]:
+        t, s = mm.rule_plan(*shape)
+        assert (t, s) in cands
+
+
+def test_workspace_bytes():
+    assert mm.workspace_bytes(64, 8192, 512, 0, (64, 16)) == 16 * 64 * 512 * 4
+    assert mm.workspace_bytes(4096, 4096, 4096, 0, (256, 1)) == 0
+    # split clamps to the number of k-blocks: K=64 with BK=32 has 2 blocks
+    assert mm.workspace_bytes(64, 64, 64, 0, (64, 16)) == 2 * 64 * 64 * 4
+    assert mm.workspace_bytes(0, 64, 64, 0, (64, 2)) == 0
+
+
+def test_planner_tiers(monkeypatch):
+    lib = mm.lib()
+    lib.maxmulsk_cache_clear()
+    monkeypatch.delenv("MAXMULSK_PLAN", raising=False)
+    plan, src = mm.choose_plan(300, 700, 500)
+    assert src == 3 and plan == mm.rule_plan(300, 700, 500)
+    assert lib.maxmulsk_cache_put(300, 700, 500, 0, mm.Plan(128, 4)) == mm.OK
+    plan, src = mm.choose_plan(300, 700, 500)
+    assert src == 1 and plan == (128, 4)
+    # other modes are separate keys
+    assert mm.choose_plan(300, 700, 500, 1)[1] == 3
+    monkeypatch.setenv("MAXMULSK_PLAN", "64,2")
+    assert mm.choose_plan(300, 700, 500) == ((64, 2), 0)
+    for bad in ["64", "64,2,3", "63,2", "64,99", "x,y", ""]:
+        monkeypatch.setenv("MAXMULSK_PLAN", bad)
+        assert mm.choose_plan(300, 700, 500)[1] == 1  # malformed override ignored
+    lib.maxmulsk_cache_clear()
+    assert lib.maxmulsk_cache_put(1, 1, 1, 0, mm.Plan(65, 1)) == mm.ERR_UNSUPPORTED
+
+
+def test_cache_file_round_trip(tmp_path):
+    lib = mm.lib()
+    lib.maxmulsk_cache_clear()
+    entries = {(64, 8192, 512, 0): (64, 16), (4096, 4096, 4096, 1): (256, 1), (7, 9, 11, 2): (128, 2)}
+    for (M, K, N, mode), p in entries.items():
+        assert lib.maxmulsk_cache_put(M, K, N, mode, mm.Plan(*p)) == mm.OK
+    path = str(tmp_path / "tune.bin").encode()
+    assert lib.maxmulsk_cache_save(path) == mm.OK
+    lib.maxmulsk_cache_clear()
+    assert lib.maxmulsk_cache_size() == 0
+    assert lib.maxmulsk_cache_load(path) == mm.OK
+    assert lib.maxmulsk_cache_size() == 3
+    for (M, K, N, mode), p in entries.items():
+        assert mm.choose_plan(M, K, N, mode) == (p, 1)
+    # a corrupted or foreign file is refused and leaves the cache unchanged
+    raw = bytearray(open(path, "rb").read())
+    raw[8] = 99  # version
+    bad = tmp_path / "bad.bin"
+    bad.write_bytes(bytes(raw))
+    assert lib.maxmulsk_cache_load(str(bad).encode()) == mm.ERR_UNSUPPORTED
+    raw = bytearray(open(path, "rb").read())
+    raw[16:22] = b"gfx942"
+    bad.write_bytes(bytes(raw))
+    assert lib.maxmulsk_cache_load(str(bad).encode()) == mm.ERR_UNSUPPORTED
+    assert lib.maxmulsk_cache_load(b"/nonexistent/x.bin") == mm.ERR_DEVICE
+    lib.maxmulsk_cache_clear()
+
+
+# ---- LDS layout ------------------------------------------------------------------
+
+# (mode, rows, bk) for every operand tile the kernels stage: tile edges 64/128/256,
+# plain and split3 at bk 32, split6 at bk 32 and 16 (256x256 tiles).
+LAYOUTS = [(m, r, bk) for m in (0, 1, 2) for r in (64, 128, 256) for bk in (32,)] + [(2, 256, 16)]
+LU = {0: 4, 1: 8, 2: 8}
+TERMS = {0: 1, 1: 2, 2: 3}
+
+
+def offsets(mode, op, rows, bk):
+    cpr = bk // LU[mode]
+    out = np.full((TERMS[mode], rows, cpr), -1, np.int64)
+    for t in range(TERMS[mode]):
+        for r in range(rows):
+            for c in range(cpr):
+                out[t, r, c] = mm.lds_chunk_offset(mode, op, rows, bk, r, c, t)
+    return out
+
+
+@pytest.mark.parametrize("mode,rows,bk", LAYOUTS)
+@pytest.mark.parametrize("op", [0, 1])
+def test_lds_layout_is_a_bijection_onto_the_stage(mode, rows, bk, op):
+    off = offsets(mode, op, rows, bk)
+    assert off.min() >= 0
+    total = TERMS[mode] * rows * bk * (4 if mode == 0 else 2)
+    assert off.max() + 16 <= total
+    assert np.all(off % 16 == 0)
+    assert len(np.unique(off)) == off.size == total // 16
+
+
+@pytest.mark.parametrize("mode,rows,bk", LAYOUTS)
+def test_lds_layout_known_answer(mode, rows, bk):
+    """A[i][k] = 1000*i + k staged into an LDS image, then read back through the
+    MFMA fragment pattern: lane l of fragment (block, k-group) must see row
+    block*32 + l%32 and k = kgroup*2*LU + (l//32)*LU + e (test_sme.cpp:265-273)."""
+    lu = LU[mode]
+    cpr = bk // lu
+    op = 0
+    img = np.full(TERMS[mode] * rows * bk * 2, -999.0)  # one slot per element
+    off = offsets(mode, op, rows, bk)
+    for t in range(TERMS[mode]):
+        for r in range(rows):
+            for c in range(cpr):
+                base = off[t, r, c] // 16 * lu
+                for e in range(lu):
+                    img[base + e] = 1000 * r + c * lu + e + 1e6 * t
+    kgroups = cpr // 2
+    for t in range(TERMS[mode]):
+        for blk in range(rows // 32):
+            for kg in range(kgroups):
+                for lane in range(64):
+                    r = blk * 32 + lane % 32
+                    c = 2 * kg + lane // 32
+                    base = mm.lds_chunk_offset(mode, op, rows, bk, r, c, t) // 16 * lu
+                    got = img[base:base + lu]
+                    want = [1000 * r + kg * 2 * lu + (lane // 32) * lu + e + 1e6 * t for e in range(lu)]
+                    assert list(got) == want
+
+
+def _worst_conflict(offs, lanes_per_cycle, bytes_per_cycle):
+    worst = 1
+    for g in range(0, 64, lanes_per_cycle):
+        slots = {}
+        for o in offs[g:g + lanes_per_cycle]:
+            if o is None:
+                continue
+            s = (o % bytes_per_cycle) // 16
+            slots[s] = slots.get(s, 0) + 1
+        if slots:
+            worst = max(worst, max(slots.values()))
+    return worst
+
+
+@pytest.mark.parametrize("mode,rows,bk", LAYOUTS)
+def test_lds_access_patterns_are_bank_conflict_free(mode, rows, bk):
+    """Simulates the kernels' writer and reader lane patterns and checks every
+    group of 8 lanes (128-byte LDS cycle) and 16 lanes (256-byte cycle) hits
+    distinct 16-byte bank groups."""
+    lu = LU[mode]
+    cpr = bk // lu
+    insts = []
+    # A writer: unit u = tid + 256*i -> (row u // cpr, chunk u % cpr)
+    total = rows * cpr
+    for i in range((total + 255) // 256):
+        for w in range(4):
+            lanes = []
+            for l in range(64):
+                u = w * 64 + l + 256 * i
+                lanes.append(None if u >= total else mm.lds_chunk_offset(mode, 0, rows, bk, u // cpr, u % cpr, 0))
+            insts.append(lanes)
+    # B writer: unit u -> (n4 = u % (rows/4), chunk = u // (rows/4)); store j writes column 4*n4+j
+    nu = rows // 4
+    total = cpr * nu
+    for i in range((total + 255) // 256):
+        for j in range(4):
+            for w in range(4):
+                lanes = []
+                for l in range(64):
+                    u = w * 64 + l + 256 * i
+                    lanes.append(None if u >= total else
+                                 mm.lds_chunk_offset(mode, 1, rows, bk, 4 * (u % nu) + j, u // nu, 0))
+                insts.append(lanes)
+    # readers of both operands
+    for op in (0, 1):
+        for blk in range(rows // 32):
+            for kg in range(cpr // 2):
+                insts.append([mm.lds_chunk_offset(mode, op, rows, bk, blk * 32 + l % 32, 2 * kg + l // 32, 0)
+                              for l in range(64)])
+    for lanes in insts:
+        assert _worst_conflict(lanes, 8, 128) == 1
+        assert _worst_conflict(lanes, 16, 256) == 1
+
+
+def test_product_input_generator_matches_oracle():
+    from maxmulsk_amd import inputs
+    from oracle import oracle as orc
+    assert np.array_equal(inputs.uniform(9, (123, 45)), orc.uniform(9, (123, 45)))
+    assert np.array_equal(inputs.log_uniform(9, (77,)), orc.log_uniform(9, (77,)))

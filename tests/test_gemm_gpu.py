@@ -1,0 +1,190 @@
+"""GPU parity of the FP32-MFMA and split-BF16 GEMM paths against the oracle,
+through the C ABI. Shapes are M x N x K as in the workload list; the ABI takes
+(M, K, N)."""
+import numpy as np
+import pytest
+
+import maxmulsk_amd as mm
+from oracle import oracle as orc
+from tests.gpu_util import SENTINEL, Guarded, cols_check, full_check, gpu_gemm, rows_check, torch_cuda
+
+pytestmark = pytest.mark.gpu
+
+ALL_PLANS = [(t, s) for t in (64, 128, 256) for s in (1, 2, 4, 8, 16)]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("shape", [(64, 512, 256), (67, 200, 130), (200, 96, 257), (1, 40, 1)])
+def test_small_integers_bit_exact_every_plan(mode, shape):
+    """Integers |x| <= 8 are exact in BF16 and every partial sum is exact in FP32,
+    so every plan, tile edge and split must reproduce the integer product exactly."""
+    M, K, N = shape
+    A = orc.small_ints(1000 + M, (M, K), 8)
+    B = orc.small_ints(2000 + N, (K, N), 8)
+    exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float32)
+    for plan in ALL_PLANS:
+        Cg = gpu_gemm(A, B, mode=mode, plan=plan)
+        bad = np.argwhere(Cg != exact)
+        assert bad.size == 0, f"mode {mode} plan {plan}: {len(bad)} wrong, first {bad[:4].tolist()}"
+
+
+# Plain FP32 MFMA on the measured shapes (M x N x K), checked against FP64 within
+# |C - C64| <= K * 2^-24 * |A||B| (gamma = 1).
+PLAIN_SHAPES = [
+    (1024, 1024, 1024), (2048, 2048, 2048), (4096, 4096, 4096),
+    (64, 512, 8192), (128, 512, 32768), (64, 24576, 1536), (64, 32768, 512),
+    (1023, 1025, 999), (67, 513, 8191),
+]
+
+
+@pytest.mark.parametrize("mnk", PLAIN_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_plain_within_fp32_bound(mnk):
+    M, N, K = mnk
+    A = orc.uniform(11, (M, K))
+    B = orc.uniform(12, (K, N))
+    Cg = gpu_gemm(A, B, mode=0, entry="v4")
+    assert not np.any(Cg == np.float32(SENTINEL))
+    if M * N * K <= 2 ** 31:
+        e = full_check(A, B, Cg)
+    else:
+        e = rows_check(A, B, Cg)
+        e2 = cols_check(A, B, Cg)
+        assert e2["bound_ratio"] <= 1.0, e2
+    assert e["nonfinite"] == 0
+    assert e["bound_ratio"] <= 1.0, e
+    print(f"plain {M}x{N}x{K}: {e}")
+
+
+@pytest.mark.parametrize("plan", ALL_PLANS)
+def test_plain_ragged_every_plan(plan):
+    M, N, K = 1023, 1025, 999
+    A = orc.uniform(21, (M, K))
+    B = orc.uniform(22, (K, N))
+    Cg = gpu_gemm(A, B, mode=0, plan=plan)
+    e = full_check(A, B, Cg)
+    assert e["bound_ratio"] <= 1.0, (plan, e)
+
+
+SPLIT_SHAPES = [(2048, 2048, 2048), (4096, 4096, 4096), (64, 7168, 16384), (4096, 24576, 1536),
+                (1023, 1025, 999), (67, 513, 8191)]
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("dist", ["uniform", "logu"])
+@pytest.mark.parametrize("mnk", SPLIT_SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_split_within_fp32_bound(mode, dist, mnk):
+    M, N, K = mnk
+    gen = orc.uniform if dist == "uniform" else orc.log_uniform
+    A = gen(31, (M, K))
+    B = gen(32, (K, N))
+    Cg = gpu_gemm(A, B, mode=mode)
+    if M * N * K <= 2 ** 31:
+        e = full_check(A, B, Cg)
+    else:
+        e = rows_check(A, B, Cg, nrows=24)
+    assert e["nonfinite"] == 0
+    print(f"mode {mode} {dist} {M}x{N}x{K}: {e}")
+    # split6 is FP32-level on any input; split3 drops lo*lo (<= 2^-16 |a||b| per
+    # product), inside the K*u32 bound for these K on both distributions
+    assert e["bound_ratio"] <= 1.0, e
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("plan", [(64, 1), (128, 4), (256, 1), (256, 16)])
+def test_split_matches_exact_split_emulation(mode, plan):
+    """The GPU result must equal the exact FP64 sum of the mode's BF16 cross
+    products up to FP32 accumulation: |C - S64| <= P*K*u32*|A||B|, P products."""
+    M, N, K = 200, 300, 640
+    A = orc.uniform(41, (M, K))
+    B = orc.log_uniform(42, (K, N), -4, 4)
+    Cg = gpu_gemm(A, B, mode=mode, plan=plan)
+    S64 = orc.split_gemm_f64(A, B, mode)
+    ab = orc.absgemm_f64(A, B)
+    P = 3 if mode == 1 else 6
+    d = np.abs(Cg.astype(np.float64) - S64)
+    assert np.all(d <= P * K * orc.U32 * ab), float(np.max(d / ab))
+    # and it is closer to the split value than the split value is to FP64 for split3
+    if mode == 1:
+        C64 = orc.gemm_f64(A, B)
+        assert np.max(np.abs(S64 - C64) / ab) > 0  # split3 is not exact
+
+
+def test_v3_v4_and_host_pointers_agree():
+    M, N, K = 300, 257, 515
+    A = orc.uniform(51, (M, K))
+    B = orc.uniform(52, (K, N))
+    torch_cuda()
+    C3 = gpu_gemm(A, B, entry="v3")
+    C4 = gpu_gemm(A, B, entry="v4")
+    Ch = np.full((M, N), SENTINEL, np.float32)
+    assert mm.sgemm_v4(A, B, Ch, M, K, N) == mm.OK, mm.last_error()
+    Ch3 = np.full((M, N), SENTINEL, np.float32)
+    assert mm.sgemm_v3(A, B, Ch3, M, K, N) == mm.OK, mm.last_error()
+    # same plan, same kernel: bit-identical
+    assert np.array_equal(C4, Ch)
+    assert np.array_equal(C3, Ch3)
+    for Cx in (C3, C4, Ch):
+        assert full_check(A, B, Cx)["bound_ratio"] <= 1.0
+
+
+def test_repeat_call_with_new_B_same_buffers():
+    """check_nb.cpp:61-66: nothing stale may survive between calls."""
+    torch = torch_cuda()
+    M, N, K = 130, 300, 2000
+    A = orc.uniform(61, (M, K))
+    B1 = orc.uniform(62, (K, N))
+    B2 = orc.uniform(63, (K, N))
+    dA, dB, dC = Guarded(M * K, A), Guarded(K * N, B1), Guarded(M * N, SENTINEL)
+    for plan in [(64, 16), (128, 4), (256, 1)]:
+        for B in (B1, B2):
+            dB.view.copy_(torch.from_numpy(B.reshape(-1)))
+            assert mm.sgemm_ex(dA.ptr, dB.ptr, dC.ptr, M, K, N, plan=plan) == mm.OK
+            torch.cuda.synchronize()
+            assert full_check(A, B, dC.numpy((M, N)))["bound_ratio"] <= 1.0
+    assert dC.guards_intact()
+
+
+def test_caller_workspace_entry_point():
+    torch = torch_cuda()
+    M, N, K = 64, 512, 8192
+    A = orc.uniform(71, (M, K))
+    B = orc.uniform(72, (K, N))
+    plan = (64, 16)
+    need = mm.workspace_bytes(M, K, N, 0, plan)
+    assert need == 16 * M * N * 4
+    ws = torch.empty(need // 4 + 64, dtype=torch.float32, device="cuda")
+    dA, dB, dC = Guarded(M * K, A), Guarded(K * N, B), Guarded(M * N, SENTINEL)
+    # short workspace: ERR_ALLOC, C untouched
+    assert mm.sgemm_ws(dA.ptr, dB.ptr, dC.ptr, M, K, N, 0, plan, ws.data_ptr(), need - 4) == mm.ERR_ALLOC
+    torch.cuda.synchronize()
+    assert np.all(dC.numpy((M, N)) == np.float32(SENTINEL))
+    # misaligned workspace: ERR_ALLOC
+    assert mm.sgemm_ws(dA.ptr, dB.ptr, dC.ptr, M, K, N, 0, plan, ws.data_ptr() + 4, need) == mm.ERR_ALLOC
+    assert mm.sgemm_ws(dA.ptr, dB.ptr, dC.ptr, M, K, N, 0, plan, ws.data_ptr(), need) == mm.OK
+    torch.cuda.synchronize()
+    assert full_check(A, B, dC.numpy((M, N)))["bound_ratio"] <= 1.0
+    # a plan without split-K needs no workspace
+    assert mm.workspace_bytes(M, K, N, 0, (64, 1)) == 0
+    assert mm.sgemm_ws(dA.ptr, dB.ptr, dC.ptr, M, K, N, 0, (64, 1), None, 0) == mm.OK
+    torch.cuda.synchronize()
+    assert full_check(A, B, dC.numpy((M, N)))["bound_ratio"] <= 1.0
+
+
+def test_async_on_a_stream_and_k_zero():
+    torch = torch_cuda()
+    M, N, K = 256, 256, 256
+    A = orc.uniform(81, (M, K))
+    B = orc.uniform(82, (K, N))
+    dA, dB, dC = Guarded(M * K, A), Guarded(K * N, B), Guarded(M * N, SENTINEL)
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    assert mm.sgemm_ex(dA.ptr, dB.ptr, dC.ptr, M, K, N, stream=s, async_=True) == mm.OK
+    s.synchronize()
+    assert full_check(A, B, dC.numpy((M, N)))["bound_ratio"] <= 1.0
+    # K == 0 on device memory: C becomes zero
+    assert mm.sgemm_v4(dA.ptr, dB.ptr, dC.ptr, M, 0, N) == mm.OK
+    torch.cuda.synchronize()
+    assert np.all(dC.numpy((M, N)) == 0.0) and dC.guards_intact()
+    # host pointers cannot be asynchronous
+    Ch = np.zeros((M, N), np.float32)
+    assert mm.sgemm_ex(A, B, Ch, M, K, N, async_=True) == mm.ERR_UNSUPPORTED
