@@ -129,17 +129,38 @@ maxmulsk_status maxmulsk_sgemm_grouped(const maxmulsk_gemm* problems, size_t n,
                                        const maxmulsk_options* opt);
 
 /* A prepared batch: plans, work list and workspace are built once and the
- * persistent launch is captured in a hipGraph; each launch replays the graph.
- * The pointers inside the problems are baked in. */
+ * launch sequence (counter reset, persistent kernel, split-K reduce) is captured
+ * in a hipGraph; each launch replays the graph. The pointers inside the
+ * problems are baked in. plans may be NULL (grouped policy per problem:
+ * grouped autotuner cache, else the grouped rule; see
+ * maxmulsk_grouped_default_plan), or hold one plan
+ * per problem. In a grouped launch a 256-wide tile on more than 128 rows runs
+ * on the 128 x 256 geometry. */
 typedef struct maxmulsk_grouped_plan_s* maxmulsk_grouped_plan;
 maxmulsk_status maxmulsk_grouped_create(const maxmulsk_gemm* problems, size_t n,
-                                        maxmulsk_mode mode, void* stream,
-                                        maxmulsk_grouped_plan* out);
+                                        maxmulsk_mode mode, const maxmulsk_plan* plans,
+                                        void* stream, maxmulsk_grouped_plan* out);
 maxmulsk_status maxmulsk_grouped_launch(maxmulsk_grouped_plan plan, void* stream);
-/* Work-list statistics: number of tiles (work items) and algorithmic FLOPs. */
+/* Work-list statistics: number of work items (tile x k-slice), algorithmic
+ * FLOPs (2MNK summed over problems) and whether the launch replays a graph. */
 maxmulsk_status maxmulsk_grouped_info(maxmulsk_grouped_plan plan, size_t* work_items,
                                       double* flops, int* used_graph);
 void maxmulsk_grouped_destroy(maxmulsk_grouped_plan plan);
+
+/* Plan a problem gets inside a grouped launch when none is given: the grouped
+ * autotuner's choice for its shape class if there is one, else the grouped
+ * rule (tile edge from N, K sliced into pieces of at most 4096). */
+maxmulsk_plan maxmulsk_grouped_default_plan(size_t M, size_t K, size_t N, maxmulsk_mode mode);
+
+/* Grouped autotuner: shape class = (M rounded up to 64/128/256/512+, N, K).
+ * Coordinate search over the candidate set, one class at a time (heaviest
+ * classes first, at most max_classes of them; <= 0 means all), timing the whole
+ * grouped launch (median of reps replays). The per-class winners are stored in
+ * the process-wide grouped cache and returned per problem in plans_out (n
+ * entries, may be NULL); *best_ms receives the final launch time (may be NULL). */
+maxmulsk_status maxmulsk_grouped_autotune(const maxmulsk_gemm* problems, size_t n,
+                                          maxmulsk_mode mode, int max_classes, int reps,
+                                          maxmulsk_plan* plans_out, float* best_ms);
 
 /* ---- autotuner (two-stage, correctness-gated; counterpart of the blocking
  * sweep bench/results/2026-09-13/source/bench_nb_sweep.cpp:4-18) ---------- */

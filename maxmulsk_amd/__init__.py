@@ -72,10 +72,13 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_choose_plan": (C.c_int, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), C.POINTER(C.c_int)]),
     "maxmulsk_rule_plan": (Plan, [_SIZE, _SIZE, _SIZE, C.c_int]),
     "maxmulsk_sgemm_grouped": (C.c_int, [C.POINTER(Gemm), _SIZE, C.POINTER(Options)]),
-    "maxmulsk_grouped_create": (C.c_int, [C.POINTER(Gemm), _SIZE, C.c_int, _P, C.POINTER(_P)]),
+    "maxmulsk_grouped_create": (C.c_int, [C.POINTER(Gemm), _SIZE, C.c_int, C.POINTER(Plan), _P, C.POINTER(_P)]),
     "maxmulsk_grouped_launch": (C.c_int, [_P, _P]),
     "maxmulsk_grouped_info": (C.c_int, [_P, C.POINTER(_SIZE), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "maxmulsk_grouped_destroy": (None, [_P]),
+    "maxmulsk_grouped_default_plan": (Plan, [_SIZE, _SIZE, _SIZE, C.c_int]),
+    "maxmulsk_grouped_autotune": (C.c_int, [C.POINTER(Gemm), _SIZE, C.c_int, C.c_int, C.c_int,
+                                            C.POINTER(Plan), C.POINTER(C.c_float)]),
     "maxmulsk_num_candidates": (C.c_int, []),
     "maxmulsk_candidate": (Plan, [C.c_int]),
     "maxmulsk_autotune": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.c_int, C.c_int,
@@ -218,3 +221,60 @@ def make_gemms(problems: Sequence[tuple[Any, Any, Any, int, int, int]]) -> Any:
     for i, (a, b, c, m, k, n) in enumerate(problems):
         arr[i] = Gemm(_ptr(a), _ptr(b), _ptr(c), m, k, n)
     return arr
+
+
+class GroupedPlan:
+    """A prepared grouped launch (maxmulsk_grouped_create); launch() replays its graph."""
+
+    def __init__(self, problems: Sequence[tuple[Any, Any, Any, int, int, int]], mode: int = MODE_PLAIN,
+                 plans: Sequence[tuple[int, int]] | None = None):
+        self._gemms = make_gemms(problems)
+        self.n = len(problems)
+        pl = None
+        if plans is not None:
+            arr = (Plan * self.n)(*[Plan(*p) for p in plans])
+            pl = arr
+        h = C.c_void_p()
+        check(lib().maxmulsk_grouped_create(self._gemms, self.n, mode, pl, None, C.byref(h)))
+        self.handle = h
+
+    def launch(self, stream: Any = None) -> None:
+        check(lib().maxmulsk_grouped_launch(self.handle, _stream_ptr(stream)))
+
+    def info(self) -> dict:
+        items = C.c_size_t()
+        flops = C.c_double()
+        graph = C.c_int()
+        check(lib().maxmulsk_grouped_info(self.handle, C.byref(items), C.byref(flops), C.byref(graph)))
+        return {"work_items": items.value, "flops": flops.value, "graph": bool(graph.value)}
+
+    def close(self) -> None:
+        if self.handle:
+            lib().maxmulsk_grouped_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sgemm_grouped(problems: Sequence[tuple[Any, Any, Any, int, int, int]], mode: int = MODE_PLAIN,
+                  stream: Any = None) -> int:
+    gemms = make_gemms(problems)
+    opt = Options(mode=mode, plan=Plan(0, 0), stream=_stream_ptr(stream), async_=0, workspace=None,
+                  workspace_bytes=0)
+    return lib().maxmulsk_sgemm_grouped(gemms, len(problems), C.byref(opt))
+
+
+def grouped_default_plan(M: int, K: int, N: int, mode: int = MODE_PLAIN) -> tuple[int, int]:
+    return lib().maxmulsk_grouped_default_plan(M, K, N, mode).as_tuple()
+
+
+def grouped_autotune(problems, mode: int = MODE_PLAIN, max_classes: int = 0, reps: int = 3):
+    gemms = make_gemms(problems)
+    out = (Plan * len(problems))()
+    best = C.c_float()
+    check(lib().maxmulsk_grouped_autotune(gemms, len(problems), mode, max_classes, reps, out, C.byref(best)))
+    return [p.as_tuple() for p in out], best.value

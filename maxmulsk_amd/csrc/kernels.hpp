@@ -56,31 +56,33 @@ struct GroupProblem {
     const float* A;
     const float* B;
     float* C;      // final output
-    float* P;      // split-K partials (nullptr when splits == 1)
+    float* P;      // split-K partials, splits x M x N (nullptr when splits == 1)
     long long M, N, K;
     long long k_len;
     int splits;
-    int cfg;       // tile geometry index
+    int cfg;       // tile geometry index into kTileCfgs
     int tiles_m, tiles_n;
     int vec_ab, vec_c;
     int item_begin;   // first work item of this problem in the flattened list
     int reduce_begin; // first reduce block of this problem (split problems only)
 };
 
-// Persistent kernel over the flattened (problem, tile, k-slice) work list;
-// work items are claimed from an atomic counter that starts at 0 (*counter must
-// be zero on entry; the kernel leaves it at total + grid). Each geometry gets its
-// own persistent kernel; `cfg` selects which problems it serves via the item
-// ranges in items_of_cfg.
+// Persistent kernel over the flattened (problem, k-slice, tile) work list. Each
+// workgroup claims items from *counter (must be zero on entry) until the list
+// is exhausted; item_problem maps an item to its problem.
 struct GroupLaunch {
     const GroupProblem* problems;
-    int num_problems;
-    const int* item_problem;  // work item -> problem index (items sorted by problem)
+    const int* item_problem;
     int total_items;
     unsigned int* counter;
 };
-hipError_t launch_grouped(int mode, int cfg, const GroupLaunch& g, int grid, hipStream_t stream);
-// Reduce for every split problem: reduce block -> problem via block_problem.
+hipError_t launch_grouped(int mode, const GroupLaunch& g, int grid, hipStream_t stream);
+// Largest tile geometry the persistent kernel serves.
+constexpr int kGroupedMaxCfg = 4;
+// Workgroups of the persistent kernel that fit on one CU at once.
+int grouped_blocks_per_cu(int mode);
+// Split-K reduce of every split problem: reduce block b serves problem
+// block_problem[b], elements [(b - reduce_begin) * 1024, +1024) of its M x N.
 hipError_t launch_grouped_reduce(const GroupProblem* problems, const int* block_problem,
                                  int total_blocks, hipStream_t stream);
 // Zero-fill (graph-safe replacement for hipMemsetAsync).

@@ -8,12 +8,7 @@ bool selector_pick(size_t, size_t, size_t, int, maxmulsk_plan*) { return false; 
 }
 #define NB(...) return fail(MAXMULSK_ERR_UNSUPPORTED, "not built yet")
 extern "C" {
-maxmulsk_status maxmulsk_sgemm_grouped(const maxmulsk_gemm*, size_t, const maxmulsk_options*) { NB(); }
-maxmulsk_status maxmulsk_grouped_create(const maxmulsk_gemm*, size_t, maxmulsk_mode, void*, maxmulsk_grouped_plan*) { NB(); }
-maxmulsk_status maxmulsk_grouped_launch(maxmulsk_grouped_plan, void*) { NB(); }
-maxmulsk_status maxmulsk_grouped_info(maxmulsk_grouped_plan, size_t*, double*, int*) { NB(); }
-void maxmulsk_grouped_destroy(maxmulsk_grouped_plan) {}
-maxmulsk_status maxmulsk_autotune(const float*, const float*, float*, size_t, size_t, size_t, maxmulsk_mode, int, int, float*, maxmulsk_plan*) { NB(); }
+
 size_t maxmulsk_mlp_num_params(void) { return 0; }
 maxmulsk_status maxmulsk_mlp_forward(const float*, const float*, int, float*, float*, float*, void*) { NB(); }
 maxmulsk_status maxmulsk_mlp_backward(const float*, const float*, int, const float*, const float*, const float*, const float*, const float*, float*, float*, void*) { NB(); }

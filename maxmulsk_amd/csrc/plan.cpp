@@ -34,13 +34,14 @@ int cfg_for(int tile, size_t M) {
     }
 }
 
-bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* out) {
+bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* out, int max_cfg) {
     if (mode < 0 || mode > 2) return false;
     if (M == 0 || K == 0 || N == 0) return false;
     // Grid and index arithmetic inside the kernels is 32-bit for tile counts.
     if (M > 0x7fffffffULL || N > 0x7fffffffULL || K > 0x7fffffffULL) return false;
-    const int cfg = cfg_for(p.tile, M);
+    int cfg = cfg_for(p.tile, M);
     if (cfg < 0) return false;
+    if (cfg > max_cfg) cfg = max_cfg;
     if (p.split_k < 1 || p.split_k > kMaxSplit) return false;
     const TileCfg& tc = kTileCfgs[cfg];
     Resolved r{};
