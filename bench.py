@@ -58,7 +58,7 @@ def main() -> int:
     dev = torch.device("cuda")
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
-    only = set(args.only.split(",")) if args.only else {"main", "narrow"}
+    only = set(args.only.split(",")) if args.only else {"main", "narrow", "grouped"}
 
     def upload(x):
         return torch.from_numpy(x).to(dev)
@@ -153,6 +153,10 @@ def main() -> int:
             del dA, dB, dC
         torch.cuda.empty_cache()
 
+    grouped = None
+    if "grouped" in only:
+        grouped = bench_grouped(torch, mm, inputs, args, log)
+
     cpu = None
     if not args.skip_cpu:
         cpu = cpu_baseline(args.cpu_seconds)
@@ -193,10 +197,59 @@ def main() -> int:
                                   "frac": out["main"]["plain"]["tflops"] / FP32_PEAK}
     if shapes:
         line["narrow_m"] = shapes
+    if grouped:
+        line["grouped"] = grouped
     if cpu:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
     return 0
+
+
+def bench_grouped(torch, mm, inputs, args, log) -> dict:
+    """256 independent GEMMs in one persistent launch replayed from a hipGraph:
+    M ~ U{1..512}, N in {512, 2112, 4096, 7168}, K in {512, 1536, 7168, 16384}
+    (drawn from one seed). A and B of every problem are slices of one random
+    pool (read-only inputs may alias); every C is its own buffer."""
+    rng = np.random.default_rng(2024)
+    specs = [(int(rng.integers(1, 513)), int(rng.choice([512, 1536, 7168, 16384])),
+              int(rng.choice([512, 2112, 4096, 7168]))) for _ in range(256)]
+    pool_n = 16384 * 7168 + 4096
+    pool = torch.from_numpy(inputs.uniform(9, (pool_n,))).cuda()
+    outs, probs = [], []
+    for M, K, N in specs:
+        oa = int(rng.integers(0, pool_n - M * K)) & ~3
+        ob = int(rng.integers(0, pool_n - K * N)) & ~3
+        c = torch.empty(M * N, dtype=torch.float32, device="cuda")
+        outs.append(c)
+        probs.append((pool.data_ptr() + 4 * oa, pool.data_ptr() + 4 * ob, c.data_ptr(), M, K, N))
+    flops = sum(2.0 * M * K * N for M, K, N in specs)
+    alg_bytes = sum(4.0 * (M * K + K * N + M * N) for M, K, N in specs)
+    res = {"problems": 256, "total_gflop": flops / 1e9, "total_alg_gb": alg_bytes / 1e9}
+    stream = torch.cuda.current_stream()
+
+    def time_plan(gp, steps):
+        for _ in range(2):
+            gp.launch(stream)
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        for _ in range(steps):
+            gp.launch(stream)
+        t1.record(stream)
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / steps
+
+    for mode_name, mode in (("plain", 0), ("split3", 1)):
+        gp = mm.GroupedPlan(probs, mode=mode)
+        ms = time_plan(gp, 10)
+        info = gp.info()
+        gp.close()
+        res[mode_name] = {"ms": ms, "tflops": flops / (ms * 1e-3) / 1e12, "work_items": info["work_items"],
+                          "graph": info["graph"]}
+        log(f"grouped 256 problems {mode_name}: {ms:.3f} ms {res[mode_name]['tflops']:.1f} TFLOP/s "
+            f"items {info['work_items']} graph {info['graph']}")
+    return res
 
 
 def cpu_baseline(seconds: float) -> dict:

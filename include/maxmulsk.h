@@ -236,6 +236,11 @@ const char* maxmulsk_last_error(void);
  * tile of `rows` rows (BM or BN) and depth bk, in the given mode; -1 when out
  * of range. */
 int maxmulsk_lds_chunk_offset(int mode, int op, int rows, int bk, int row, int c, int term);
+/* Staging geometry of tile geometry cfg (0..maxmulsk_num_tile_geometries()-1)
+ * in a mode: out[0..5] = BM, BN, BK, threads per workgroup, columns per B
+ * staging unit, waves. Returns 0, or -1 for an unknown cfg or mode. */
+int maxmulsk_num_tile_geometries(void);
+int maxmulsk_tile_info(int cfg, int mode, int* out);
 
 #ifdef __cplusplus
 }

@@ -97,6 +97,8 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_device_count": (C.c_int, []),
     "maxmulsk_last_error": (C.c_char_p, []),
     "maxmulsk_lds_chunk_offset": (C.c_int, [C.c_int] * 7),
+    "maxmulsk_num_tile_geometries": (C.c_int, []),
+    "maxmulsk_tile_info": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int)]),
 }
 EXPORTED = tuple(_SIGS)
 
@@ -209,6 +211,18 @@ def candidates() -> list[tuple[int, int]]:
 
 def lds_chunk_offset(mode: int, op: int, rows: int, bk: int, row: int, c: int, term: int) -> int:
     return lib().maxmulsk_lds_chunk_offset(mode, op, rows, bk, row, c, term)
+
+
+def tile_info(cfg: int, mode: int) -> dict:
+    out = (C.c_int * 6)()
+    if lib().maxmulsk_tile_info(cfg, mode, out) != 0:
+        raise MaxMulSKError(f"no tile geometry {cfg} for mode {mode}")
+    return dict(zip(("bm", "bn", "bk", "threads", "b_cols", "waves"), list(out)))
+
+
+def tile_infos() -> list[dict]:
+    n = lib().maxmulsk_num_tile_geometries()
+    return [dict(tile_info(c, m), cfg=c, mode=m) for c in range(n) for m in (0, 1, 2)]
 
 
 def check(status: int) -> None:

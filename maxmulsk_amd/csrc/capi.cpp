@@ -155,3 +155,18 @@ extern "C" int maxmulsk_lds_chunk_offset(int mode, int op, int rows, int bk, int
         return -1;
     return chunk_offset(mode, op, rows, bk, row, c, term);
 }
+
+extern "C" int maxmulsk_num_tile_geometries(void) { return kNumTileCfgs; }
+
+extern "C" int maxmulsk_tile_info(int cfg, int mode, int* out) {
+    if (cfg < 0 || cfg >= kNumTileCfgs || mode < 0 || mode > 2 || !out) return -1;
+    const TileCfg& t = kTileCfgs[cfg];
+    const int waves = t.wgm * t.wgn;
+    out[0] = t.bm;
+    out[1] = t.bn;
+    out[2] = t.bk[mode];
+    out[3] = 64 * waves;
+    out[4] = waves == 8 ? 2 : 4;  // B staging unit width used by the tile kernel
+    out[5] = waves;
+    return 0;
+}

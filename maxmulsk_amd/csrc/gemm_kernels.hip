@@ -8,7 +8,7 @@
 namespace mmsk {
 
 template <int MODE, int BM, int BN, int BK, int WGM, int WGN>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmParams p) {
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmParams p) {
     using T = Tile<MODE, BM, BN, BK, WGM, WGN>;
     __shared__ __attribute__((aligned(16))) char smem[T::SMEM];
     const int ntiles = p.tiles_m * p.tiles_n;
@@ -54,7 +54,7 @@ static hipError_t launch_one(const GemmParams& p, hipStream_t stream) {
     const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
     if (grid <= 0 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, BK, c.wgm, c.wgn>), dim3((unsigned)grid),
-                       dim3(256), 0, stream, p);
+                       dim3(64 * c.wgm * c.wgn), 0, stream, p);
     return hipGetLastError();
 }
 

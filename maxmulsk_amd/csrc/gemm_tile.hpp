@@ -1,4 +1,4 @@
-// Device-side GEMM tile: one workgroup (4 waves) computes one BM x BN output
+// Device-side GEMM tile: one workgroup (4 or 8 waves) computes one BM x BN output
 // tile over one K range. Shared by the one-tile-per-workgroup kernel and the
 // grouped persistent kernel.
 //
@@ -103,9 +103,13 @@ struct Tile {
     static constexpr int CPR = BK / LU;             // 16-byte chunks per row of a k-block
     static constexpr int KG = CPR / 2;              // 2-chunk k-groups (one MFMA fragment deep)
     static constexpr int A_UNITS = BM * CPR;        // A unit: one chunk of one row
-    static constexpr int B_UNITS = CPR * (BN / 4);  // B unit: LU k-rows x 4 columns
-    static constexpr int UA = (A_UNITS + 255) / 256;
-    static constexpr int UB = (B_UNITS + 255) / 256;
+    // B unit: LU k-rows x BW columns; eight-wave tiles use 2 columns to keep the
+    // staging registers within the two-waves-per-SIMD budget.
+    static constexpr int BW = (WGM * WGN == 8) ? 2 : 4;
+    static constexpr int B_UNITS = CPR * (BN / BW);
+    static constexpr int NT = 64 * WGM * WGN;       // threads per workgroup
+    static constexpr int UA = (A_UNITS + NT - 1) / NT;
+    static constexpr int UB = (B_UNITS + NT - 1) / NT;
     static constexpr int A_BYTES = operand_bytes(MODE, BM, BK);
     static constexpr int B_BYTES = operand_bytes(MODE, BN, BK);
     static constexpr int STAGE = A_BYTES + B_BYTES;
@@ -113,13 +117,14 @@ struct Tile {
     static constexpr int WTM = BM / WGM, WTN = BN / WGN;
     static constexpr int TM = WTM / 32, TN = WTN / 32;
 
-    static_assert(WGM * WGN == 4, "four waves per workgroup");
+    static_assert(WGM * WGN == 4 || WGM * WGN == 8, "four or eight waves per workgroup");
     static_assert(BK % (2 * LU) == 0, "k-block must hold whole fragments");
     static_assert(TM >= 1 && TN >= 1, "wave tile must hold a 32x32 block");
     static_assert(SMEM <= 160 * 1024, "LDS budget of one CU");
 
     f32x4 ra[UA][LU / 4];  // A unit i: LU consecutive k of one row
-    f32x4 rb[UB][LU];      // B unit i: row e holds 4 consecutive columns
+    using bvec = float __attribute__((ext_vector_type(BW)));
+    bvec rb[UB][LU];       // B unit i: row e holds BW consecutive columns
     f32x16 acc[TM][TN];
 
     __device__ __forceinline__ void load(const float* __restrict__ A, const float* __restrict__ B,
@@ -130,8 +135,8 @@ struct Tile {
         if (fast) {
 #pragma unroll
             for (int i = 0; i < UA; ++i) {
-                const int u = tid + 256 * i;
-                if (A_UNITS % 256 == 0 || u < A_UNITS) {
+                const int u = tid + NT * i;
+                if (A_UNITS % NT == 0 || u < A_UNITS) {
                     const int row = u / CPR, c = u % CPR;
                     const float* src = A + (m0 + row) * lda + k0 + c * LU;
 #pragma unroll
@@ -141,20 +146,20 @@ struct Tile {
             }
 #pragma unroll
             for (int i = 0; i < UB; ++i) {
-                const int u = tid + 256 * i;
-                if (B_UNITS % 256 == 0 || u < B_UNITS) {
-                    const int n4 = u % (BN / 4), c = u / (BN / 4);
-                    const float* src = B + (k0 + c * LU) * ldb + n0 + 4 * n4;
+                const int u = tid + NT * i;
+                if (B_UNITS % NT == 0 || u < B_UNITS) {
+                    const int nb = u % (BN / BW), c = u / (BN / BW);
+                    const float* src = B + (k0 + c * LU) * ldb + n0 + BW * nb;
 #pragma unroll
                     for (int e = 0; e < LU; ++e)
-                        rb[i][e] = *reinterpret_cast<const f32x4*>(src + e * ldb);
+                        rb[i][e] = *reinterpret_cast<const bvec*>(src + e * ldb);
                 }
             }
         } else {
 #pragma unroll
             for (int i = 0; i < UA; ++i) {
-                const int u = tid + 256 * i;
-                if (A_UNITS % 256 == 0 || u < A_UNITS) {
+                const int u = tid + NT * i;
+                if (A_UNITS % NT == 0 || u < A_UNITS) {
                     const int row = u / CPR, c = u % CPR;
                     const long long gm = m0 + row, gk = k0 + c * LU;
 #pragma unroll
@@ -167,15 +172,15 @@ struct Tile {
             }
 #pragma unroll
             for (int i = 0; i < UB; ++i) {
-                const int u = tid + 256 * i;
-                if (B_UNITS % 256 == 0 || u < B_UNITS) {
-                    const int n4 = u % (BN / 4), c = u / (BN / 4);
-                    const long long gn = n0 + 4 * n4;
+                const int u = tid + NT * i;
+                if (B_UNITS % NT == 0 || u < B_UNITS) {
+                    const int nb = u % (BN / BW), c = u / (BN / BW);
+                    const long long gn = n0 + BW * nb;
 #pragma unroll
                     for (int e = 0; e < LU; ++e) {
                         const long long gk = k0 + c * LU + e;
 #pragma unroll
-                        for (int j = 0; j < 4; ++j) {
+                        for (int j = 0; j < BW; ++j) {
                             float x = 0.f;
                             if (gk < kend && gn + j < N) x = B[gk * ldb + gn + j];
                             rb[i][e][j] = x;
@@ -190,8 +195,8 @@ struct Tile {
         const int tid = threadIdx.x;
 #pragma unroll
         for (int i = 0; i < UA; ++i) {
-            const int u = tid + 256 * i;
-            if (A_UNITS % 256 == 0 || u < A_UNITS) {
+            const int u = tid + NT * i;
+            if (A_UNITS % NT == 0 || u < A_UNITS) {
                 const int row = u / CPR, c = u % CPR;
                 if constexpr (MODE == 0) {
                     *reinterpret_cast<f32x4*>(stage + chunk_offset(MODE, 0, BM, BK, row, c, 0)) = ra[i][0];
@@ -215,12 +220,12 @@ struct Tile {
         char* bstage = stage + A_BYTES;
 #pragma unroll
         for (int i = 0; i < UB; ++i) {
-            const int u = tid + 256 * i;
-            if (B_UNITS % 256 == 0 || u < B_UNITS) {
-                const int n4 = u % (BN / 4), c = u / (BN / 4);
+            const int u = tid + NT * i;
+            if (B_UNITS % NT == 0 || u < B_UNITS) {
+                const int nb = u % (BN / BW), c = u / (BN / BW);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int n = 4 * n4 + j;
+                for (int j = 0; j < BW; ++j) {
+                    const int n = BW * nb + j;
                     if constexpr (MODE == 0) {
                         const f32x4 v = {rb[i][0][j], rb[i][1][j], rb[i][2][j], rb[i][3][j]};
                         *reinterpret_cast<f32x4*>(bstage + chunk_offset(MODE, 1, BN, BK, n, c, 0)) = v;

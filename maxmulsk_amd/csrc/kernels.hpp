@@ -7,8 +7,9 @@
 
 namespace mmsk {
 
-// Tile geometries compiled for every mode. Four waves (256 threads) per
-// workgroup, laid out wgm x wgn over the output tile.
+// Tile geometries compiled for every mode: wgm x wgn waves over the output
+// tile. The 256 x 256 geometry runs eight waves (two per SIMD) so one wave's
+// staging work overlaps the other's MFMAs; the others run four.
 struct TileCfg {
     int bm, bn;
     int wgm, wgn;
@@ -22,7 +23,7 @@ constexpr TileCfg kTileCfgs[kNumTileCfgs] = {
     {64, 256, 1, 4, {32, 32, 32}},
     {128, 128, 2, 2, {32, 32, 32}},
     {128, 256, 2, 2, {32, 32, 32}},
-    {256, 256, 2, 2, {32, 32, 16}},
+    {256, 256, 2, 4, {32, 32, 16}},
 };
 
 // One launch of the tiled kernel: an output grid of tiles_m x tiles_n tiles,

@@ -18,8 +18,9 @@
 //
 // The A-tile swizzle mixes the chunk index into the low bits (the A writer
 // spreads one row's chunks over consecutive lanes); the B-tile swizzle mixes
-// row bits 3-4 and the block parity (the B writer spreads columns 4 apart over
-// consecutive lanes). Neither swizzle moves a chunk out of its lane half.
+// row bits 3-4 and (row bit 4 xor block parity) into bits 0-1 and 3 (the B
+// writer spreads columns 2 or 4 apart over consecutive lanes). Neither swizzle
+// moves a chunk out of its lane half.
 #pragma once
 
 #if defined(__HIPCC__)
@@ -45,7 +46,7 @@ MMSK_HD constexpr int swizzle_slot(int op, int r, int rblk, int c, int chunks_pe
     const int pos = r + 32 * (c & 1);
     const int s = a_row_shift(chunks_per_row);
     const int mask = op == 0 ? (((c << s) ^ (((r >> s) & 1) << 3)) & 31)
-                             : (((r >> 3) & 3) | ((rblk & 1) << 3));
+                             : (((r >> 3) & 3) | ((((r >> 4) ^ rblk) & 1) << 3));
     return pos ^ mask;
 }
 

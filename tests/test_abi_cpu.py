@@ -226,37 +226,43 @@ def _worst_conflict(offs, lanes_per_cycle, bytes_per_cycle):
     return worst
 
 
-@pytest.mark.parametrize("mode,rows,bk", LAYOUTS)
-def test_lds_access_patterns_are_bank_conflict_free(mode, rows, bk):
-    """Simulates the kernels' writer and reader lane patterns and checks every
-    group of 8 lanes (128-byte LDS cycle) and 16 lanes (256-byte cycle) hits
-    distinct 16-byte bank groups."""
+def _geometries():
+    return [(g["cfg"], g["mode"]) for g in mm.tile_infos()]
+
+
+@pytest.mark.parametrize("cfg,mode", _geometries())
+def test_lds_access_patterns_are_bank_conflict_free(cfg, mode):
+    """Simulates the tile kernel's writer and reader lane patterns for every
+    geometry and mode and checks every group of 8 lanes (128-byte LDS cycle) and
+    16 lanes (256-byte cycle) hits distinct 16-byte bank groups."""
+    g = mm.tile_info(cfg, mode)
+    bm, bn, bk, nt, bw = g["bm"], g["bn"], g["bk"], g["threads"], g["b_cols"]
     lu = LU[mode]
     cpr = bk // lu
     insts = []
-    # A writer: unit u = tid + 256*i -> (row u // cpr, chunk u % cpr)
-    total = rows * cpr
-    for i in range((total + 255) // 256):
-        for w in range(4):
+    # A writer: unit u = tid + nt*i -> (row u // cpr, chunk u % cpr)
+    total = bm * cpr
+    for i in range((total + nt - 1) // nt):
+        for w in range(nt // 64):
             lanes = []
             for l in range(64):
-                u = w * 64 + l + 256 * i
-                lanes.append(None if u >= total else mm.lds_chunk_offset(mode, 0, rows, bk, u // cpr, u % cpr, 0))
+                u = w * 64 + l + nt * i
+                lanes.append(None if u >= total else mm.lds_chunk_offset(mode, 0, bm, bk, u // cpr, u % cpr, 0))
             insts.append(lanes)
-    # B writer: unit u -> (n4 = u % (rows/4), chunk = u // (rows/4)); store j writes column 4*n4+j
-    nu = rows // 4
+    # B writer: unit u -> (nb = u % (bn/bw), chunk = u // (bn/bw)); store j writes column bw*nb+j
+    nu = bn // bw
     total = cpr * nu
-    for i in range((total + 255) // 256):
-        for j in range(4):
-            for w in range(4):
+    for i in range((total + nt - 1) // nt):
+        for j in range(bw):
+            for w in range(nt // 64):
                 lanes = []
                 for l in range(64):
-                    u = w * 64 + l + 256 * i
+                    u = w * 64 + l + nt * i
                     lanes.append(None if u >= total else
-                                 mm.lds_chunk_offset(mode, 1, rows, bk, 4 * (u % nu) + j, u // nu, 0))
+                                 mm.lds_chunk_offset(mode, 1, bn, bk, bw * (u % nu) + j, u // nu, 0))
                 insts.append(lanes)
     # readers of both operands
-    for op in (0, 1):
+    for op, rows in ((0, bm), (1, bn)):
         for blk in range(rows // 32):
             for kg in range(cpr // 2):
                 insts.append([mm.lds_chunk_offset(mode, op, rows, bk, blk * 32 + l % 32, 2 * kg + l // 32, 0)
@@ -264,6 +270,16 @@ def test_lds_access_patterns_are_bank_conflict_free(mode, rows, bk):
     for lanes in insts:
         assert _worst_conflict(lanes, 8, 128) == 1
         assert _worst_conflict(lanes, 16, 256) == 1
+
+
+def test_tile_geometries():
+    infos = mm.tile_infos()
+    assert len(infos) == 18
+    for g in infos:
+        assert g["threads"] == 64 * g["waves"] and g["waves"] in (4, 8)
+        # LDS budget of one CU: two stages of both operands
+        per_elem = 4 if g["mode"] == 0 else 2 * {1: 2, 2: 3}[g["mode"]]
+        assert 2 * (g["bm"] + g["bn"]) * g["bk"] * per_elem <= 160 * 1024
 
 
 def test_product_input_generator_matches_oracle():
