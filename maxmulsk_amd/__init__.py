@@ -112,6 +112,13 @@ def lib() -> C.CDLL:
         if not os.path.exists(LIB_PATH):
             raise MaxMulSKError(f"{LIB_PATH} is not built (run __graft_entry__.build() "
                                 "or python -m maxmulsk_amd.build)")
+        # torch wheels bundle their own libamdhip64 (same soname, other file
+        # name). Loading torch first makes libmaxmulsk bind to that runtime, so
+        # the process holds one HIP runtime whichever of the two is used first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         h = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(h, name)
