@@ -3,9 +3,9 @@
 // grouped persistent kernel.
 //
 // Pipeline per BK-deep k-block (two LDS stages, one barrier per block):
-//   global -> registers for block kb+1 (issued first, in flight during MFMA)
 //   MFMA over the LDS stage of block kb
-//   registers -> (split to BF16 terms) -> the other LDS stage
+//   registers of block kb+1 -> (split to BF16 terms) -> the other LDS stage
+//   global -> registers for block kb+2 (in flight across the barrier)
 //   barrier
 // Edge tiles and the last partial k-block take the masked, element-wise load
 // path (zero fill outside M, N and the K range); full tiles take dwordx4 loads.
@@ -276,32 +276,35 @@ struct Tile {
                             acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x2f32(
                                 b[tj][s], a[ti][s], acc[ti][tj], 0, 0, 0);
             } else {
-                bf16x8 a[TERMS][TM], b[TERMS][TN];
+                // B fragments once per k-group; A fragments one 32-row block at
+                // a time, so only TERMS x (TN + 1) fragments are live.
+                bf16x8 b[TERMS][TN];
 #pragma unroll
-                for (int s = 0; s < TERMS; ++s) {
-#pragma unroll
-                    for (int ti = 0; ti < TM; ++ti)
-                        a[s][ti] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
-                            stage + chunk_offset(MODE, 0, BM, BK, wm * WTM + ti * 32 + r, c, s)));
+                for (int s = 0; s < TERMS; ++s)
 #pragma unroll
                     for (int tj = 0; tj < TN; ++tj)
                         b[s][tj] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
                             bstage + chunk_offset(MODE, 1, BN, BK, wn * WTN + tj * 32 + r, c, s)));
-                }
-                // Cross products, smallest first, all into the one FP32
-                // accumulator: split3 = lo*hi + hi*lo + hi*hi; split6 adds the
-                // mid terms (Henry, Tang & Heinecke, ARITH 2019).
-                constexpr int NP = MODE == 1 ? 3 : 6;
 #pragma unroll
-                for (int p = 0; p < NP; ++p) {
-                    const int ta = product_term(MODE, p, 0);
-                    const int tb = product_term(MODE, p, 1);
+                for (int ti = 0; ti < TM; ++ti) {
+                    bf16x8 a[TERMS];
 #pragma unroll
-                    for (int ti = 0; ti < TM; ++ti)
+                    for (int s = 0; s < TERMS; ++s)
+                        a[s] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                            stage + chunk_offset(MODE, 0, BM, BK, wm * WTM + ti * 32 + r, c, s)));
+                    // Cross products, smallest first, all into the one FP32
+                    // accumulator: split3 = lo*hi + hi*lo + hi*hi; split6 adds the
+                    // mid terms (Henry, Tang & Heinecke, ARITH 2019).
+                    constexpr int NP = MODE == 1 ? 3 : 6;
+#pragma unroll
+                    for (int p = 0; p < NP; ++p) {
+                        const int ta = product_term(MODE, p, 0);
+                        const int tb = product_term(MODE, p, 1);
 #pragma unroll
                         for (int tj = 0; tj < TN; ++tj)
-                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                                b[tb][tj], a[ta][ti], acc[ti][tj], 0, 0, 0);
+                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[tb][tj], a[ta], acc[ti][tj],
+                                                                                  0, 0, 0);
+                    }
                 }
             }
         }
@@ -340,6 +343,35 @@ struct Tile {
         }
     }
 
+    // Main loop over the k-blocks of [kbeg, kend). ALL_FAST: every block is a
+    // full, aligned block (dwordx4 loads, no bounds checks), so the steady-state
+    // body is one basic block and the scheduler can interleave the MFMAs of
+    // stage kb with the split/store of stage kb+1. Registers for block kb+2 are
+    // loaded once the store of kb+1 has consumed them, i.e. one full iteration
+    // (plus the barrier) before they are needed.
+    template <bool ALL_FAST>
+    __device__ __forceinline__ void mainloop(const float* __restrict__ A, const float* __restrict__ B,
+                                             long long M, long long N, long long lda, long long ldb,
+                                             long long m0, long long n0, long long kbeg, long long kend,
+                                             bool full_mn, char* smem) {
+        const long long nkb = (kend - kbeg + BK - 1) / BK;
+        auto fetch = [&](long long k0) {
+            load(A, B, M, N, lda, ldb, m0, n0, k0, kend, ALL_FAST || (full_mn && k0 + BK <= kend));
+        };
+        fetch(kbeg);
+        store(smem);
+        if (nkb > 1) fetch(kbeg + BK);
+        __syncthreads();
+        for (long long kb = 0; kb + 1 < nkb; ++kb) {
+            compute(smem + (kb & 1) * STAGE);
+            store(smem + ((kb + 1) & 1) * STAGE);
+            if (kb + 2 < nkb) fetch(kbeg + (kb + 2) * BK);
+            __syncthreads();
+        }
+        compute(smem + ((nkb - 1) & 1) * STAGE);
+        __syncthreads();
+    }
+
     // Whole tile: C[m0:m0+BM, n0:n0+BN] = A[m0:, kbeg:kend] * B[kbeg:kend, n0:].
     // smem holds SMEM bytes; on return every wave has passed the last barrier,
     // so the caller may reuse smem for the next tile without another barrier.
@@ -356,20 +388,10 @@ struct Tile {
                 for (int e = 0; e < 16; ++e) acc[ti][tj][e] = 0.f;
 
         const bool full_mn = (m0 + BM <= M) && (n0 + BN <= N) && vec_ab;
-        const long long nkb = (kend - kbeg + BK - 1) / BK;
-        load(A, B, M, N, lda, ldb, m0, n0, kbeg, kend, full_mn && kbeg + BK <= kend);
-        store(smem);
-        __syncthreads();
-        for (long long kb = 0; kb < nkb; ++kb) {
-            char* cur = smem + (kb & 1) * STAGE;
-            char* nxt = smem + ((kb + 1) & 1) * STAGE;
-            const bool more = kb + 1 < nkb;
-            const long long k1 = kbeg + (kb + 1) * BK;
-            if (more) load(A, B, M, N, lda, ldb, m0, n0, k1, kend, full_mn && k1 + BK <= kend);
-            compute(cur);
-            if (more) store(nxt);
-            __syncthreads();
-        }
+        if (full_mn && (kend - kbeg) % BK == 0)
+            mainloop<true>(A, B, M, N, lda, ldb, m0, n0, kbeg, kend, full_mn, smem);
+        else
+            mainloop<false>(A, B, M, N, lda, ldb, m0, n0, kbeg, kend, full_mn, smem);
         epilogue(C, M, N, ldc, m0, n0, vec_c);
     }
 };
