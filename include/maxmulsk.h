@@ -214,10 +214,20 @@ maxmulsk_status maxmulsk_mlp_backward(const float* params, const float* X, int b
 maxmulsk_status maxmulsk_mlp_adam(float* params, const float* dparams, float* m, float* v,
                                   int step, float lr, float beta1, float beta2, float eps,
                                   void* stream);
+/* Network input of a shape: (log2 M, log2 N, log2 K) / 13 and mode / 2. */
+void maxmulsk_selector_features(size_t M, size_t K, size_t N, maxmulsk_mode mode, float* out4);
 /* Installs host-side weights as the process-wide selector used by the planner
  * (tier 2); NULL uninstalls it. A plan is taken from the network only where it
- * is valid for the shape. */
+ * is valid for the shape. Planner picks are memoised per shape. */
 maxmulsk_status maxmulsk_selector_set(const float* host_params);
+/* Loads weights from a file written by maxmulsk_selector_save (magic, geometry,
+ * then the FP32 parameters, little endian) and installs them. Without an
+ * explicit call the planner loads $MAXMULSK_SELECTOR, else
+ * maxmulsk_selector.bin next to the library, on first use (if present). */
+maxmulsk_status maxmulsk_selector_load(const char* path);
+maxmulsk_status maxmulsk_selector_save(const char* path, const float* host_params);
+/* 1 when a selector is installed. */
+int maxmulsk_selector_active(void);
 /* Selector inference on the device through the dedicated one-wave kernel:
  * writes the predicted log-runtime of every candidate into pred (host array of
  * maxmulsk_num_candidates() floats) and the arg-min into *best. */

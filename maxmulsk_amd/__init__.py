@@ -93,6 +93,10 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_mlp_backward": (C.c_int, [_P, _P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P]),
     "maxmulsk_mlp_adam": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, _P]),
     "maxmulsk_selector_set": (C.c_int, [_P]),
+    "maxmulsk_selector_features": (None, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(C.c_float)]),
+    "maxmulsk_selector_load": (C.c_int, [C.c_char_p]),
+    "maxmulsk_selector_save": (C.c_int, [C.c_char_p, _P]),
+    "maxmulsk_selector_active": (C.c_int, []),
     "maxmulsk_selector_predict": (C.c_int, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(C.c_float), C.POINTER(Plan)]),
     "maxmulsk_device_count": (C.c_int, []),
     "maxmulsk_last_error": (C.c_char_p, []),
@@ -299,3 +303,27 @@ def grouped_autotune(problems, mode: int = MODE_PLAIN, max_classes: int = 0, rep
     best = C.c_float()
     check(lib().maxmulsk_grouped_autotune(gemms, len(problems), mode, max_classes, reps, out, C.byref(best)))
     return [p.as_tuple() for p in out], best.value
+
+
+def selector_features(M: int, K: int, N: int, mode: int = MODE_PLAIN) -> list[float]:
+    out = (C.c_float * 4)()
+    lib().maxmulsk_selector_features(M, K, N, mode, out)
+    return list(out)
+
+
+def selector_predict(M: int, K: int, N: int, mode: int = MODE_PLAIN) -> tuple[list[float], tuple[int, int]]:
+    n = lib().maxmulsk_num_candidates()
+    pred = (C.c_float * n)()
+    best = Plan()
+    check(lib().maxmulsk_selector_predict(M, K, N, mode, pred, C.byref(best)))
+    return list(pred), best.as_tuple()
+
+
+def autotune(A: Any, B: Any, Cm: Any, M: int, K: int, N: int, mode: int = MODE_PLAIN,
+             stage1_reps: int = 0, stage2_reps: int = 0) -> tuple[list[float], tuple[int, int]]:
+    n = lib().maxmulsk_num_candidates()
+    times = (C.c_float * n)()
+    best = Plan()
+    check(lib().maxmulsk_autotune(_ptr(A), _ptr(B), _ptr(Cm), M, K, N, mode, stage1_reps, stage2_reps,
+                                  times, C.byref(best)))
+    return list(times), best.as_tuple()

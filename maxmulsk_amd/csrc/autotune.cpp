@@ -53,20 +53,18 @@ struct Tuner {
     }
 };
 
-// Strided sample of C (at most 4096 values, always including the last element).
+// Strided sample of C: at most 4096 values at a fixed stride (one 2-D copy).
 std::vector<float> sample_c(const float* C, size_t M, size_t N, hipStream_t s) {
     const size_t total = M * N;
     const size_t count = std::min<size_t>(total, 4096);
     const size_t stride = std::max<size_t>(1, total / count);
     std::vector<float> out(count);
-    for (size_t i = 0; i < count; ++i) {
-        const size_t idx = (i + 1 == count) ? total - 1 : i * stride;
-        if (hipMemcpyAsync(&out[i], C + idx, sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess) {
-            out.clear();
-            return out;
-        }
+    if (hipMemcpy2DAsync(out.data(), sizeof(float), C, stride * sizeof(float), sizeof(float), count,
+                         hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipGetLastError();
+        out.clear();
     }
-    if (hipStreamSynchronize(s) != hipSuccess) out.clear();
     return out;
 }
 

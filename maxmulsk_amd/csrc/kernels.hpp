@@ -93,12 +93,14 @@ hipError_t launch_fill_zero_f32(float* p, size_t n, hipStream_t stream);
 // ---- MLP (learned selector) ---------------------------------------------------
 constexpr int kMlpIn = 4;
 constexpr int kMlpHidden = 64;
+int mlp_num_params(int nout);
 hipError_t launch_mlp_forward(const float* params, const float* X, int batch, int nout,
                               float* H1, float* H2, float* Y, hipStream_t stream);
+// scratch: device floats, at least 64 + 2 * batch * kMlpHidden.
 hipError_t launch_mlp_backward(const float* params, const float* X, int batch, int nout,
                                const float* H1, const float* H2, const float* Y,
                                const float* T, const float* Mask, float* dparams,
-                               float* loss, hipStream_t stream);
+                               float* loss, hipStream_t stream, float* scratch);
 hipError_t launch_mlp_adam(float* params, const float* g, float* m, float* v, int n, int step,
                            float lr, float b1, float b2, float eps, hipStream_t stream);
 hipError_t launch_mlp_infer(const float* params, const float* x, int nout, float* y,
