@@ -53,14 +53,15 @@ struct Tuner {
     }
 };
 
-// Strided sample of C: at most 4096 values at a fixed stride (one 2-D copy).
-std::vector<float> sample_c(const float* C, size_t M, size_t N, hipStream_t s) {
+// Strided sample of C: at most 4096 values at a fixed stride, gathered on the
+// device into `buf` and copied back in one piece.
+std::vector<float> sample_c(const float* C, size_t M, size_t N, float* buf, hipStream_t s) {
     const size_t total = M * N;
     const size_t count = std::min<size_t>(total, 4096);
     const size_t stride = std::max<size_t>(1, total / count);
     std::vector<float> out(count);
-    if (hipMemcpy2DAsync(out.data(), sizeof(float), C, stride * sizeof(float), sizeof(float), count,
-                         hipMemcpyDeviceToHost, s) != hipSuccess ||
+    if (launch_gather_strided(C, stride, count, buf, s) != hipSuccess ||
+        hipMemcpyAsync(out.data(), buf, count * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         (void)hipGetLastError();
         out.clear();
@@ -115,13 +116,14 @@ extern "C" maxmulsk_status maxmulsk_autotune(const float* A, const float* B, flo
         return fail(MAXMULSK_ERR_DEVICE, "event wait failed");
     }
 
-    Tuner t{A, B, C, M, K, N, (int)mode, pl.ws, nullptr, nullptr, nullptr};
-    if (hipStreamCreateWithFlags(&t.s, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&t.e0) != hipSuccess || hipEventCreate(&t.e1) != hipSuccess) {
+    if ((!pl.tune_stream && hipStreamCreateWithFlags(&pl.tune_stream, hipStreamNonBlocking) != hipSuccess) ||
+        (!pl.tune_e0 && hipEventCreate(&pl.tune_e0) != hipSuccess) ||
+        (!pl.tune_e1 && hipEventCreate(&pl.tune_e1) != hipSuccess) ||
+        (!pl.tune_sample && hipMalloc(&pl.tune_sample, 4096 * sizeof(float)) != hipSuccess)) {
         (void)hipGetLastError();
-        if (t.s) (void)hipStreamDestroy(t.s);
         return fail(MAXMULSK_ERR_DEVICE, "stream/event creation failed");
     }
+    Tuner t{A, B, C, M, K, N, (int)mode, pl.ws, pl.tune_stream, pl.tune_e0, pl.tune_e1};
     // Wait for work the caller queued on the legacy stream (the inputs).
     (void)hipDeviceSynchronize();
 
@@ -135,7 +137,7 @@ extern "C" maxmulsk_status maxmulsk_autotune(const float* A, const float* B, flo
             valid[i] = false;
             continue;
         }
-        std::vector<float> smp = sample_c(C, M, N, t.s);
+        std::vector<float> smp = sample_c(C, M, N, pl.tune_sample, t.s);
         if (ref.empty()) {
             bool finite = !smp.empty();
             for (float v : smp) finite = finite && std::isfinite(v);
@@ -163,12 +165,7 @@ extern "C" maxmulsk_status maxmulsk_autotune(const float* A, const float* B, flo
         stage1[i] = median(passes[i]);
         best1 = std::min(best1, stage1[i]);
     }
-    if (best1 >= 1e30f) {
-        (void)hipStreamDestroy(t.s);
-        (void)hipEventDestroy(t.e0);
-        (void)hipEventDestroy(t.e1);
-        return fail(MAXMULSK_ERR_DEVICE, "no candidate ran correctly");
-    }
+    if (best1 >= 1e30f) return fail(MAXMULSK_ERR_DEVICE, "no candidate ran correctly");
 
     // Stage 2: leaders.
     std::vector<int> order;
@@ -200,9 +197,6 @@ extern "C" maxmulsk_status maxmulsk_autotune(const float* A, const float* B, flo
             best_i = i;
         }
     }
-    (void)hipStreamDestroy(t.s);
-    (void)hipEventDestroy(t.e0);
-    (void)hipEventDestroy(t.e1);
     if (best_i < 0) return fail(MAXMULSK_ERR_DEVICE, "autotune produced no timing");
     if (times_ms)
         for (int i = 0; i < kNumCandidates; ++i) times_ms[i] = final_t[i];

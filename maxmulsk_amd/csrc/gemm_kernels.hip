@@ -112,4 +112,18 @@ hipError_t launch_fill_zero_f32(float* p, size_t n, hipStream_t stream) {
     return launch_fill_zero_u32(reinterpret_cast<unsigned int*>(p), n, stream);
 }
 
+__global__ __launch_bounds__(256) void gather_strided_kernel(const float* __restrict__ src, size_t stride,
+                                                             size_t count, float* __restrict__ dst) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) dst[i] = src[i * stride];
+}
+
+hipError_t launch_gather_strided(const float* src, size_t stride, size_t count, float* dst,
+                                 hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_strided_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, src,
+                       stride, count, dst);
+    return hipGetLastError();
+}
+
 }  // namespace mmsk

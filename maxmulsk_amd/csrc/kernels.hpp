@@ -89,6 +89,9 @@ hipError_t launch_grouped_reduce(const GroupProblem* problems, const int* block_
 // Zero-fill (graph-safe replacement for hipMemsetAsync).
 hipError_t launch_fill_zero_u32(unsigned int* p, size_t n, hipStream_t stream);
 hipError_t launch_fill_zero_f32(float* p, size_t n, hipStream_t stream);
+// dst[i] = src[i * stride] for i < count.
+hipError_t launch_gather_strided(const float* src, size_t stride, size_t count, float* dst,
+                                 hipStream_t stream);
 
 // ---- MLP (learned selector) ---------------------------------------------------
 constexpr int kMlpIn = 4;

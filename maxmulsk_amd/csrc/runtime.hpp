@@ -46,6 +46,11 @@ struct DevicePools {
     bool ws_event_recorded = false;
     void* stage = nullptr;
     size_t stage_cap = 0;
+    // Stream and events of the autotuner, created on first use and kept (a
+    // stream holds a hardware queue; creating one per tuning call is costly).
+    hipStream_t tune_stream = nullptr;
+    hipEvent_t tune_e0 = nullptr, tune_e1 = nullptr;
+    float* tune_sample = nullptr;  // 4096 floats: gathered sample of C
 };
 DevicePools& pools(int dev);
 

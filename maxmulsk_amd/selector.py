@@ -39,7 +39,7 @@ def draw_shapes(n: int, seed: int, lo: int = 1, hi: int = 8192) -> tuple[np.ndar
     return mnk, modes
 
 
-def collect(mnk: np.ndarray, modes: np.ndarray, stage1: int = 2, stage2: int = 4, log=print) -> np.ndarray:
+def collect(mnk: np.ndarray, modes: np.ndarray, stage1: int = 3, stage2: int = 6, log=print) -> np.ndarray:
     """Autotuner times (ms) per shape and candidate; -1 where a candidate failed."""
     import torch
     M_max, N_max, K_max = (int(mnk[:, i].max()) for i in range(3))

@@ -43,8 +43,8 @@ def test_forward_loss_gradient_adam_match_oracle():
     assert abs(float(tr.loss.item()) - L64) <= 1e-5 * max(1.0, L64)
     g = tr.g.cpu().numpy()
     assert np.max(np.abs(g - g64)) <= 1e-4 * np.max(np.abs(g64)) + 1e-7
-    # every parameter block receives a gradient
-    assert np.count_nonzero(g) > 0.9 * g.size
+    # the same parameters receive no gradient (dead ReLU units) on both sides
+    assert np.mean((g != 0) == (g64 != 0)) > 0.99
     mm.check(mm.lib().maxmulsk_mlp_adam(tr.p.data_ptr(), tr.g.data_ptr(), tr.m.data_ptr(), tr.v.data_ptr(), 1,
                                         1e-3, 0.9, 0.999, 1e-8, None))
     torch.cuda.synchronize()
