@@ -1,0 +1,102 @@
+"""What the trained selector gains over the planner without it, on the held-out
+shapes of the committed autotuner table (maxmulsk_amd/selector_dataset.npz,
+timed on an MI355X by maxmulsk_amd.selector). Regret = t(chosen)/t(best) - 1,
+read from the table.
+
+The CPU test evaluates the committed weights with the FP64 MLP oracle; the GPU
+test runs the same choices through the library's one-wave inference kernel."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import maxmulsk_amd as mm
+from maxmulsk_amd import selector as sel
+from oracle import mlp_oracle as mo
+from oracle import oracle as orc
+
+PKG = os.path.dirname(mm.__file__)
+DATA = np.load(os.path.join(PKG, "selector_dataset.npz"))
+WEIGHTS = os.path.join(PKG, "maxmulsk_selector.bin")
+
+
+def weights():
+    raw = open(WEIGHTS, "rb").read()
+    assert raw[:8] == b"MMSKMLP1"
+    geo = np.frombuffer(raw[8:20], np.uint32)
+    assert tuple(geo) == (4, 64, 15)
+    return np.frombuffer(raw[20:], np.float32)
+
+
+def heldout():
+    n = int(DATA["n_train"])
+    return DATA["mnk"][n:], DATA["modes"][n:], DATA["times"][n:]
+
+
+def v4c_analog(M, K, N):
+    """The reference's v4c shape rule (sme/v4/sme-1x4-v4c-dispatch.cpp:17-22) on
+    this candidate set: its large profile (M >= 2048, N > 512) -> the largest
+    tile without split; otherwise the small tile with K cut into Kc = 2048
+    slices (the reference's outer K panel)."""
+    Mc, Nc, Kc = orc.v4c_choose(M, K, N)
+    if Mc == 32:
+        return (256, 1)
+    return (64, int(min(16, max(1, 2 ** int(np.ceil(np.log2(max(1, K / Kc))))))))
+
+
+def test_dataset_shape_and_report():
+    mnk, modes, times = DATA["mnk"], DATA["modes"], DATA["times"]
+    assert mnk.shape == (5500, 3) and times.shape == (5500, 15) and int(DATA["n_train"]) == 5000
+    assert mnk.min() >= 1 and mnk.max() <= 8192 and set(np.unique(modes)) <= {0, 1, 2}
+    assert np.mean(times > 0) > 0.99
+    rep = json.load(open(os.path.join(PKG, "selector_report.json")))
+    assert rep["heldout"]["mlp"]["mean"] <= 0.05
+
+
+def test_selector_gain_on_heldout_cpu():
+    p = weights()
+    mnk, modes, times = heldout()
+    X = np.array([mm.selector_features(int(M), int(K), int(N), int(md))
+                  for (M, N, K), md in zip(mnk.tolist(), modes.tolist())], np.float32)
+    _, _, Y = mo.forward(p, X, 15)
+    mlp = Y.argmin(1)
+    r_mlp = sel.regret_of(mlp, times)
+    r_rule = sel.regret_of(np.array([sel.cand_index(mm.rule_plan(int(M), int(K), int(N), int(md)))
+                                     for (M, N, K), md in zip(mnk.tolist(), modes.tolist())]), times)
+    r_fixed = sel.regret_of(np.full(len(mnk), sel.cand_index(sel.FIXED_PLAN)), times)
+    r_v4c = sel.regret_of(np.array([sel.cand_index(v4c_analog(int(M), int(K), int(N)))
+                                    for (M, N, K) in mnk.tolist()]), times)
+    m = {k: float(np.nanmean(v)) for k, v in
+         (("mlp", r_mlp), ("rule", r_rule), ("fixed", r_fixed), ("v4c", r_v4c))}
+    print("held-out mean regret:", m)
+    # target: within 5% of the autotuner's best on average, and better than
+    # every method without the network
+    assert m["mlp"] <= 0.05
+    assert m["mlp"] < m["rule"] and m["mlp"] < m["fixed"] and m["mlp"] < m["v4c"]
+
+
+@pytest.mark.gpu
+def test_selector_gain_on_heldout_inference_kernel():
+    from tests.gpu_util import torch_cuda
+    torch_cuda()
+    lib = mm.lib()
+    mm.check(lib.maxmulsk_selector_load(WEIGHTS.encode()))
+    try:
+        mnk, modes, times = heldout()
+        choice = []
+        for (M, N, K), md in zip(mnk.tolist(), modes.tolist()):
+            _, best = mm.selector_predict(M, K, N, md)
+            choice.append(sel.cand_index(best))
+        r = sel.regret_of(np.array(choice), times)
+        p = weights()
+        X = np.array([mm.selector_features(int(M), int(K), int(N), int(md))
+                      for (M, N, K), md in zip(mnk.tolist(), modes.tolist())], np.float32)
+        _, _, Y = mo.forward(p, X, 15)
+        # the kernel's picks equal the oracle's, up to near-ties of the predictions
+        srt = np.sort(Y, 1)
+        clear = (srt[:, 1] - srt[:, 0]) > 1e-3
+        assert np.array_equal(np.array(choice)[clear], Y.argmin(1)[clear])
+        assert float(np.nanmean(r)) <= 0.05
+    finally:
+        lib.maxmulsk_selector_set(None)
