@@ -276,8 +276,9 @@ struct Tile {
                             acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x2f32(
                                 b[tj][s], a[ti][s], acc[ti][tj], 0, 0, 0);
             } else {
-                // B fragments once per k-group; A fragments one 32-row block at
-                // a time, so only TERMS x (TN + 1) fragments are live.
+                // B fragments once per k-group; A fragments two 32-row blocks
+                // at a time (register budget of two waves per SIMD), products
+                // outer, so MFMAs into one accumulator are 2*TN apart.
                 bf16x8 b[TERMS][TN];
 #pragma unroll
                 for (int s = 0; s < TERMS; ++s)
@@ -285,13 +286,16 @@ struct Tile {
                     for (int tj = 0; tj < TN; ++tj)
                         b[s][tj] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
                             bstage + chunk_offset(MODE, 1, BN, BK, wn * WTN + tj * 32 + r, c, s)));
+                constexpr int TP = TM >= 2 ? 2 : 1;
 #pragma unroll
-                for (int ti = 0; ti < TM; ++ti) {
-                    bf16x8 a[TERMS];
+                for (int t0 = 0; t0 < TM; t0 += TP) {
+                    bf16x8 a[TERMS][TP];
 #pragma unroll
-                    for (int s = 0; s < TERMS; ++s)
-                        a[s] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
-                            stage + chunk_offset(MODE, 0, BM, BK, wm * WTM + ti * 32 + r, c, s)));
+                    for (int i = 0; i < TP; ++i)
+#pragma unroll
+                        for (int s = 0; s < TERMS; ++s)
+                            a[s][i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                                stage + chunk_offset(MODE, 0, BM, BK, wm * WTM + (t0 + i) * 32 + r, c, s)));
                     // Cross products, smallest first, all into the one FP32
                     // accumulator: split3 = lo*hi + hi*lo + hi*hi; split6 adds the
                     // mid terms (Henry, Tang & Heinecke, ARITH 2019).
@@ -301,9 +305,11 @@ struct Tile {
                         const int ta = product_term(MODE, p, 0);
                         const int tb = product_term(MODE, p, 1);
 #pragma unroll
-                        for (int tj = 0; tj < TN; ++tj)
-                            acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[tb][tj], a[ta], acc[ti][tj],
-                                                                                  0, 0, 0);
+                        for (int i = 0; i < TP; ++i)
+#pragma unroll
+                            for (int tj = 0; tj < TN; ++tj)
+                                acc[t0 + i][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                                    b[tb][tj], a[ta][i], acc[t0 + i][tj], 0, 0, 0);
                     }
                 }
             }

@@ -46,6 +46,31 @@ struct GemmParams {
 
 hipError_t launch_gemm(int mode, int cfg, const GemmParams& p, hipStream_t stream);
 
+// ---- packed path (256 x 256 geometry) -------------------------------------------
+// k-block depth of the packed GEMM and k values per packed fragment.
+constexpr int kPackedBK = 16;
+constexpr int kPackedCfg = 5;
+constexpr int packed_kgk(int mode) { return mode == 0 ? 8 : 16; }
+
+struct PackedParams {
+    const char* PA;
+    const char* PB;
+    long long planeA, planeB;  // bytes per BF16 term (or the FP32 copy)
+    int kgtA, kgtB;            // fragments per row block along K
+    float* C;
+    long long M, N, ldc;
+    long long c_split_stride;
+    long long kblocks;            // k-blocks of kPackedBK in the packed operands
+    long long kblocks_per_split;
+    int splits, tiles_m, tiles_n;
+    int vec_c;
+};
+// Packs X (element (row, k) at X[row * ld_row + k * ld_k], rows < R, k < K)
+// into rblocks x kgroups fragments per term, zero-padded.
+hipError_t launch_pack(int mode, const float* X, long long R, long long K, long long ld_row, long long ld_k,
+                       void* dst, int rblocks, int kgroups, hipStream_t stream);
+hipError_t launch_gemm_packed(int mode, const PackedParams& p, hipStream_t stream);
+
 // C[i] = sum_{s < splits} P[s * stride + i] for i < M*N, summed in s order.
 hipError_t launch_splitk_reduce(const float* P, float* C, long long mn, int splits,
                                 long long stride, hipStream_t stream);

@@ -48,7 +48,8 @@ bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* 
     r.cfg = cfg;
     r.bm = tc.bm;
     r.bn = tc.bn;
-    r.bk = tc.bk[mode];
+    r.packed = (cfg == kPackedCfg);
+    r.bk = r.packed ? kPackedBK : tc.bk[mode];
     const long long kblocks = ((long long)K + r.bk - 1) / r.bk;
     long long per = (kblocks + p.split_k - 1) / p.split_k;  // k-blocks per slice
     if (per < 1) per = 1;
@@ -58,11 +59,19 @@ bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* 
     r.tiles_n = (int)((N + r.bn - 1) / r.bn);
     const long long grid = (long long)r.tiles_m * r.tiles_n * r.splits;
     if (grid > 0x7fffffffLL) return false;
-    r.ws_bytes = 0;
-    if (r.splits > 1) {
-        const unsigned long long bytes = (unsigned long long)r.splits * M * N * sizeof(float);
-        r.ws_bytes = (size_t)((bytes + 255) / 256 * 256);
+    auto up = [](unsigned long long b) { return (size_t)((b + 255) / 256 * 256); };
+    r.partial_bytes = r.splits > 1 ? up((unsigned long long)r.splits * M * N * sizeof(float)) : 0;
+    r.kblocks = kblocks;
+    r.kgroups = 0;
+    r.pack_a_bytes = r.pack_b_bytes = 0;
+    if (r.packed) {
+        const int terms = mode == 0 ? 1 : (mode == 1 ? 2 : 3);
+        r.kgroups = (int)(kblocks * (kPackedBK / packed_kgk(mode)));
+        const unsigned long long frag = 1024ULL * (unsigned long long)r.kgroups * terms;
+        r.pack_a_bytes = up(frag * (unsigned long long)r.tiles_m * (r.bm / 32));
+        r.pack_b_bytes = up(frag * (unsigned long long)r.tiles_n * (r.bn / 32));
     }
+    r.ws_bytes = r.partial_bytes + r.pack_a_bytes + r.pack_b_bytes;
     *out = r;
     return true;
 }

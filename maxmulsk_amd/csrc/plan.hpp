@@ -31,7 +31,13 @@ struct Resolved {
     int splits;          // K slices actually used (<= plan.split_k)
     long long k_len;     // K elements per slice, a multiple of bk
     int tiles_m, tiles_n;
-    size_t ws_bytes;     // split-K partials (0 when splits == 1)
+    size_t ws_bytes;     // split-K partials + packed operands
+    // Packed path (256 x 256 geometry): operands rewritten once into MFMA
+    // fragments in the workspace, after the partials.
+    bool packed;
+    long long kblocks;   // packed k-blocks (kPackedBK deep)
+    int kgroups;         // fragments per row block along K
+    size_t partial_bytes, pack_a_bytes, pack_b_bytes;
 };
 
 // Counterpart of classify (sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:410-423)

@@ -125,33 +125,70 @@ bool grow(void** buf, size_t* cap, size_t bytes) {
 
 hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, size_t K, size_t N,
                            int mode, const Resolved& r, void* ws, hipStream_t stream) {
-    GemmParams p{};
-    p.A = A;
-    p.B = B;
-    p.M = (long long)M;
-    p.N = (long long)N;
-    p.K = (long long)K;
-    p.lda = (long long)K;
-    p.ldb = (long long)N;
-    p.ldc = (long long)N;
-    p.k_len = r.k_len;
-    p.splits = r.splits;
-    p.tiles_m = r.tiles_m;
-    p.tiles_n = r.tiles_n;
-    p.vec_ab = (K % 4 == 0 && N % 4 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0);
-    if (r.splits > 1) {
-        p.C = static_cast<float*>(ws);
-        p.c_split_stride = (long long)(M * N);
-        p.vec_c = (N % 4 == 0);
+    char* wsb = static_cast<char*>(ws);
+    float* partials = r.splits > 1 ? reinterpret_cast<float*>(wsb) : nullptr;
+    hipError_t e = hipSuccess;
+    if (r.packed) {
+        char* pa = wsb + r.partial_bytes;
+        char* pb = pa + r.pack_a_bytes;
+        const int terms = mode == 0 ? 1 : (mode == 1 ? 2 : 3);
+        const int rbA = r.tiles_m * (r.bm / 32), rbB = r.tiles_n * (r.bn / 32);
+        e = launch_pack(mode, A, (long long)M, (long long)K, (long long)K, 1, pa, rbA, r.kgroups, stream);
+        if (e == hipSuccess)
+            e = launch_pack(mode, B, (long long)N, (long long)K, 1, (long long)N, pb, rbB, r.kgroups, stream);
+        if (e != hipSuccess) return e;
+        PackedParams p{};
+        p.PA = pa;
+        p.PB = pb;
+        p.planeA = (long long)r.pack_a_bytes / terms;
+        p.planeB = (long long)r.pack_b_bytes / terms;
+        p.kgtA = p.kgtB = r.kgroups;
+        p.M = (long long)M;
+        p.N = (long long)N;
+        p.ldc = (long long)N;
+        p.kblocks = r.kblocks;
+        p.kblocks_per_split = r.k_len / r.bk;
+        p.splits = r.splits;
+        p.tiles_m = r.tiles_m;
+        p.tiles_n = r.tiles_n;
+        if (r.splits > 1) {
+            p.C = partials;
+            p.c_split_stride = (long long)(M * N);
+            p.vec_c = (N % 4 == 0);
+        } else {
+            p.C = C;
+            p.c_split_stride = 0;
+            p.vec_c = (N % 4 == 0 && ((uintptr_t)C & 15) == 0);
+        }
+        e = launch_gemm_packed(mode, p, stream);
     } else {
-        p.C = C;
-        p.c_split_stride = 0;
-        p.vec_c = (N % 4 == 0 && ((uintptr_t)C & 15) == 0);
+        GemmParams p{};
+        p.A = A;
+        p.B = B;
+        p.M = (long long)M;
+        p.N = (long long)N;
+        p.K = (long long)K;
+        p.lda = (long long)K;
+        p.ldb = (long long)N;
+        p.ldc = (long long)N;
+        p.k_len = r.k_len;
+        p.splits = r.splits;
+        p.tiles_m = r.tiles_m;
+        p.tiles_n = r.tiles_n;
+        p.vec_ab = (K % 4 == 0 && N % 4 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0);
+        if (r.splits > 1) {
+            p.C = partials;
+            p.c_split_stride = (long long)(M * N);
+            p.vec_c = (N % 4 == 0);
+        } else {
+            p.C = C;
+            p.c_split_stride = 0;
+            p.vec_c = (N % 4 == 0 && ((uintptr_t)C & 15) == 0);
+        }
+        e = launch_gemm(mode, r.cfg, p, stream);
     }
-    hipError_t e = launch_gemm(mode, r.cfg, p, stream);
     if (e != hipSuccess || r.splits == 1) return e;
-    return launch_splitk_reduce(static_cast<const float*>(ws), C, (long long)(M * N), r.splits,
-                                (long long)(M * N), stream);
+    return launch_splitk_reduce(partials, C, (long long)(M * N), r.splits, (long long)(M * N), stream);
 }
 
 maxmulsk_status run_gemm(const GemmCall& c) {

@@ -99,7 +99,14 @@ def test_candidates_and_rule_plan():
 
 def test_workspace_bytes():
     assert mm.workspace_bytes(64, 8192, 512, 0, (64, 16)) == 16 * 64 * 512 * 4
-    assert mm.workspace_bytes(4096, 4096, 4096, 0, (256, 1)) == 0
+    # 256-wide tiles on more than 128 rows take the packed path: the workspace
+    # holds A and B rewritten as fragments (FP32 copy, or 2 / 3 BF16 terms),
+    # padded to whole 256-row tiles and 16-deep k-blocks
+    assert mm.workspace_bytes(4096, 4096, 4096, 0, (256, 1)) == 2 * 4096 * 4096 * 4
+    assert mm.workspace_bytes(4096, 4096, 4096, 1, (256, 1)) == 2 * 2 * 4096 * 4096 * 2
+    assert mm.workspace_bytes(4096, 4096, 4096, 2, (256, 1)) == 2 * 3 * 4096 * 4096 * 2
+    assert mm.workspace_bytes(300, 20, 300, 1, (256, 1)) == 2 * (2 * 512 * 32 * 2)
+    assert mm.workspace_bytes(4096, 4096, 4096, 0, (128, 1)) == 0
     # split clamps to the number of k-blocks: K=64 with BK=32 has 2 blocks
     assert mm.workspace_bytes(64, 64, 64, 0, (64, 16)) == 2 * 64 * 64 * 4
     assert mm.workspace_bytes(0, 64, 64, 0, (64, 2)) == 0
