@@ -99,6 +99,25 @@ maxmulsk_status maxmulsk_sgemm_ws(const float* A, const float* B, float* C,
                                   maxmulsk_mode mode, const maxmulsk_plan* plan,
                                   void* ws, size_t ws_bytes, void* stream);
 
+/* Pack once, multiply many: an operand rewritten into the MFMA fragment
+ * layout (split into its BF16 terms in the split modes), owned by the library.
+ * Operand 0 is an A (rows = M, row-major M x K), operand 1 a B (rows = N,
+ * row-major K x N). Packed operands feed the 256 x 256 tile path, the one
+ * maxmulsk_sgemm_ex takes for 256-wide plans on more than 128 rows, which packs
+ * both operands on every call; a caller that reuses an operand (weights)
+ * packs it once. Counterpart of the reference's B panel packed once per
+ * (Kc, Nc) block (sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:498-503). Device
+ * pointers only. */
+typedef struct maxmulsk_packed_s* maxmulsk_packed;
+maxmulsk_status maxmulsk_pack(const float* X, size_t rows, size_t K, int operand, maxmulsk_mode mode,
+                              void* stream, maxmulsk_packed* out);
+/* C = A*B from two packed operands of the same mode and K (M = A's rows,
+ * N = B's rows). opt->mode and opt->plan.tile are ignored; opt->plan.split_k
+ * (0 = 1) splits K; workspace, stream and async as in maxmulsk_sgemm_ex. */
+maxmulsk_status maxmulsk_sgemm_packed(maxmulsk_packed A, maxmulsk_packed B, float* C,
+                                      const maxmulsk_options* opt);
+void maxmulsk_packed_destroy(maxmulsk_packed p);
+
 /* The plan the planner would run for this shape and mode, without running
  * anything (counterpart of SMEKernels1x4V4C::choose,
  * sme/v4/sme-1x4-v4c-dispatch.hpp:61). source receives 0 = environment

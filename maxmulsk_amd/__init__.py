@@ -69,6 +69,9 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_sgemm_ex": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.POINTER(Options)]),
     "maxmulsk_workspace_bytes": (C.c_size_t, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan)]),
     "maxmulsk_sgemm_ws": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), _P, _SIZE, _P]),
+    "maxmulsk_pack": (C.c_int, [_P, _SIZE, _SIZE, C.c_int, C.c_int, _P, C.POINTER(_P)]),
+    "maxmulsk_sgemm_packed": (C.c_int, [_P, _P, _P, C.POINTER(Options)]),
+    "maxmulsk_packed_destroy": (None, [_P]),
     "maxmulsk_choose_plan": (C.c_int, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), C.POINTER(C.c_int)]),
     "maxmulsk_rule_plan": (Plan, [_SIZE, _SIZE, _SIZE, C.c_int]),
     "maxmulsk_sgemm_grouped": (C.c_int, [C.POINTER(Gemm), _SIZE, C.POINTER(Options)]),
@@ -283,6 +286,39 @@ class GroupedPlan:
             self.close()
         except Exception:
             pass
+
+
+class Packed:
+    """An operand packed once (maxmulsk_pack): operand 0 = A (rows x K,
+    row-major), 1 = B (K x rows, row-major). Multiply with sgemm_packed."""
+
+    def __init__(self, X: Any, rows: int, K: int, operand: int, mode: int | str = MODE_PLAIN,
+                 stream: Any = None):
+        if isinstance(mode, str):
+            mode = MODES[mode]
+        h = C.c_void_p()
+        check(lib().maxmulsk_pack(_ptr(X), rows, K, operand, mode, _stream_ptr(stream), C.byref(h)))
+        self.handle = h
+        self.rows, self.K, self.operand, self.mode = rows, K, operand, mode
+
+    def close(self) -> None:
+        if self.handle:
+            lib().maxmulsk_packed_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sgemm_packed(A: Packed, B: Packed, Cm: Any, split_k: int = 1, stream: Any = None,
+                 async_: bool = False, workspace: Any = None, workspace_bytes: int = 0) -> int:
+    """C (A.rows x B.rows) = A*B from packed operands; returns the status."""
+    opt = Options(mode=A.mode, plan=Plan(0, split_k), stream=_stream_ptr(stream), async_=1 if async_ else 0,
+                  workspace=_ptr(workspace) if workspace is not None else None, workspace_bytes=workspace_bytes)
+    return lib().maxmulsk_sgemm_packed(A.handle, B.handle, _ptr(Cm), C.byref(opt))
 
 
 def sgemm_grouped(problems: Sequence[tuple[Any, Any, Any, int, int, int]], mode: int = MODE_PLAIN,

@@ -123,6 +123,26 @@ bool grow(void** buf, size_t* cap, size_t bytes) {
     return true;
 }
 
+maxmulsk_status pool_workspace_acquire(DevicePools& pl, size_t bytes, hipStream_t stream, void** ws) {
+    if (!grow(&pl.ws, &pl.ws_cap, bytes)) return fail(MAXMULSK_ERR_ALLOC, "workspace allocation of %zu bytes failed", bytes);
+    if (!pl.ws_event && hipEventCreateWithFlags(&pl.ws_event, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MAXMULSK_ERR_DEVICE, "event creation failed");
+    }
+    if (pl.ws_event_recorded && hipStreamWaitEvent(stream, pl.ws_event, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MAXMULSK_ERR_DEVICE, "stream wait failed");
+    }
+    *ws = pl.ws;
+    return MAXMULSK_OK;
+}
+
+hipError_t pool_workspace_release(DevicePools& pl, hipStream_t stream) {
+    const hipError_t e = hipEventRecord(pl.ws_event, stream);
+    pl.ws_event_recorded = (e == hipSuccess);
+    return e;
+}
+
 hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, size_t K, size_t N,
                            int mode, const Resolved& r, void* ws, hipStream_t stream) {
     char* wsb = static_cast<char*>(ws);
@@ -293,23 +313,12 @@ maxmulsk_status run_gemm(const GemmCall& c) {
     void* ws = c.ws;
     const bool pooled = r.ws_bytes > 0 && !ws;
     if (pooled) {
-        if (!grow(&pl.ws, &pl.ws_cap, r.ws_bytes)) return fail(MAXMULSK_ERR_ALLOC, "workspace allocation of %zu bytes failed", r.ws_bytes);
-        if (!pl.ws_event && hipEventCreateWithFlags(&pl.ws_event, hipEventDisableTiming) != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(MAXMULSK_ERR_DEVICE, "event creation failed");
-        }
-        if (pl.ws_event_recorded && hipStreamWaitEvent(c.stream, pl.ws_event, 0) != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(MAXMULSK_ERR_DEVICE, "stream wait failed");
-        }
-        ws = pl.ws;
+        const maxmulsk_status st = pool_workspace_acquire(pl, r.ws_bytes, c.stream, &ws);
+        if (st != MAXMULSK_OK) return st;
     }
 
     hipError_t e = launch_resolved(A, B, C, c.M, c.K, c.N, c.mode, r, ws, c.stream);
-    if (e == hipSuccess && pooled) {
-        e = hipEventRecord(pl.ws_event, c.stream);
-        pl.ws_event_recorded = (e == hipSuccess);
-    }
+    if (e == hipSuccess && pooled) e = pool_workspace_release(pl, c.stream);
     if (e == hipSuccess && staged && !c_dev)
         e = hipMemcpyAsync(c.C, C, c.M * c.N * sizeof(float), hipMemcpyDeviceToHost, c.stream);
     if (e == hipSuccess && !c.async) e = hipStreamSynchronize(c.stream);

@@ -58,6 +58,12 @@ DevicePools& pools(int dev);
 // device before freeing the old buffer. Returns false on allocation failure.
 bool grow(void** buf, size_t* cap, size_t bytes);
 
+// Stream-orders the pooled workspace (grown to `bytes`) behind its previous
+// use and hands it out; caller holds pl.mu. After enqueuing the work that uses
+// it, the caller records the release on the same stream.
+maxmulsk_status pool_workspace_acquire(DevicePools& pl, size_t bytes, hipStream_t stream, void** ws);
+hipError_t pool_workspace_release(DevicePools& pl, hipStream_t stream);
+
 // The GEMM launch itself (no contract checks): all pointers are device
 // pointers, `ws` holds r.ws_bytes when r.splits > 1.
 hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, size_t K, size_t N,

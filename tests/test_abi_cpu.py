@@ -72,6 +72,20 @@ def test_null_pointers_rejected():
     assert mm.sgemm_v4(None, None, None, 2, 2, 2) == mm.ERR_UNSUPPORTED
 
 
+def test_pack_api_rejects_bad_arguments_without_a_device_call():
+    lib = mm.lib()
+    h = mm.C.c_void_p(1234)
+    X = np.ones(64, np.float32)
+    assert lib.maxmulsk_pack(X.ctypes.data, 8, 8, 0, 1, None, mm.C.byref(h)) == mm.ERR_UNSUPPORTED
+    assert h.value is None  # out is cleared on failure
+    assert lib.maxmulsk_pack(None, 8, 8, 0, 1, None, mm.C.byref(h)) == mm.ERR_UNSUPPORTED
+    assert lib.maxmulsk_pack(X.ctypes.data, 8, 8, 2, 1, None, mm.C.byref(h)) == mm.ERR_UNSUPPORTED
+    assert lib.maxmulsk_pack(X.ctypes.data, 8, 8, 0, 5, None, mm.C.byref(h)) == mm.ERR_UNSUPPORTED
+    assert lib.maxmulsk_pack(X.ctypes.data, 0, 8, 0, 1, None, mm.C.byref(h)) == mm.ERR_UNSUPPORTED
+    assert lib.maxmulsk_sgemm_packed(None, None, X.ctypes.data, None) == mm.ERR_UNSUPPORTED
+    lib.maxmulsk_packed_destroy(None)
+
+
 def test_invalid_plan_rejected_before_anything_runs():
     A = np.ones((8, 8), np.float32)
     C = np.full((8, 8), 9.0, np.float32)
