@@ -42,6 +42,8 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only", default="", help="comma list of sections: main,narrow,grouped (default all)")
     ap.add_argument("--modes", default="plain,split3,split6", help="modes timed at 4096^3")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a 256 MiB device copy (reference for traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the GEMM path has no "
@@ -104,11 +106,27 @@ def main() -> int:
 
     def tune(dA, dB, dC, M, K, N, mode):
         """Fill the plan cache for this shape with the autotuner's pick, so the
-        planner's cache tier serves the timed calls."""
+        planner's cache tier serves the timed calls. A MAXMULSK_PLAN override
+        (the planner's first tier) skips tuning, e.g. under a profiler."""
+        if os.environ.get("MAXMULSK_PLAN"):
+            plan, src = mm.choose_plan(M, K, N, mode)
+            assert src == 0, "MAXMULSK_PLAN did not parse as 'tile,split'"
+            return plan
         _, best = mm.autotune(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, mode, 3, 8)
         plan, src = mm.choose_plan(M, K, N, mode)
         assert src == 1 and plan == best, (plan, src, best)
         return plan
+
+    if args.copy_calibration:
+        # A plain streaming kernel of known size (256 MiB read, 256 MiB
+        # written; torch's elementwise multiply), run three times, for scaling
+        # hardware traffic counters.
+        src_buf = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst_buf = torch.empty_like(src_buf)
+        for _ in range(3):
+            torch.mul(src_buf, 2.0, out=dst_buf)
+        torch.cuda.synchronize()
+        del src_buf, dst_buf
 
     def packed_kernel_ms(dA, dB, dC, M, K, N, mode, split, steps):
         """Average duration of the packed GEMM launch alone: both operands
@@ -149,7 +167,8 @@ def main() -> int:
             ms, wall = timed(lambda: call(dA, dB, dC, M, K, N, mode, None, ws), args.steps, args.warmup)
             e = err_fp64(dA, dB, dC, K)
             tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
-            r = {"tflops": tf, "ms": ms, "plan": list(plan), "plan_source": "autotune cache", "errors": e}
+            source = "MAXMULSK_PLAN" if os.environ.get("MAXMULSK_PLAN") else "autotune cache"
+            r = {"tflops": tf, "ms": ms, "plan": list(plan), "plan_source": source, "errors": e}
             if plan[0] == 256:
                 # the 256-wide path packs both operands on every call; time its
                 # GEMM kernel alone on pre-packed operands
@@ -161,6 +180,27 @@ def main() -> int:
             res[mode_name] = r
             log(f"4096^3 {mode_name}: {ms:.4f} ms  {tf:.1f} TFLOP/s  plan {plan}  "
                 f"kernel {r['gemm_kernel_ms']:.4f} ms  err {e}")
+        if "split3" in res:
+            # The drop-in case: maxmulsk_sgemm_v4 on host (pageable) arrays, so
+            # each call stages A and B to the device and C back over PCIe.
+            A_h, B_h = dA.cpu().numpy(), dB.cpu().numpy()
+            C_h = np.empty((M, N), np.float32)
+            os.environ["MAXMULSK_MODE"] = "split3"
+            try:
+                mm.check(mm.sgemm_v4(A_h, B_h, C_h, M, K, N))
+                ts = []
+                for _ in range(3):
+                    t = time.perf_counter()
+                    mm.check(mm.sgemm_v4(A_h, B_h, C_h, M, K, N))
+                    ts.append(time.perf_counter() - t)
+            finally:
+                del os.environ["MAXMULSK_MODE"]
+            hms = float(np.median(ts)) * 1e3
+            res["split3"]["host_pointer_ms"] = hms
+            res["split3"]["host_pointer_tflops"] = 2.0 * M * N * K / (hms * 1e-3) / 1e12
+            log(f"4096^3 split3 through maxmulsk_sgemm_v4 on host arrays (PCIe staging): {hms:.2f} ms "
+                f"{res['split3']['host_pointer_tflops']:.1f} TFLOP/s")
+            del A_h, B_h, C_h
         out["main"] = res
         # the headline: the fastest split mode that stays inside the FP32 bound
         ok = [m for m in ("split3", "split6") if m in res and res[m]["errors"]["within_fp32_bound"]]
