@@ -201,11 +201,25 @@ def main() -> int:
             log(f"4096^3 split3 through maxmulsk_sgemm_v4 on host arrays (PCIe staging): {hms:.2f} ms "
                 f"{res['split3']['host_pointer_tflops']:.1f} TFLOP/s")
             del A_h, B_h, C_h
+        # Context: the vendor library (torch.matmul -> hipBLASLt) on the same
+        # operands, FP32 (no reduced-precision math) and plain BF16.
+        torch.backends.cuda.matmul.allow_tf32 = False
+        vendor = {}
+        for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
+            a, b = dA.to(dt), dB.to(dt)
+            c = torch.empty((M, N), dtype=dt, device=dev)
+            vms, _ = timed(lambda: torch.matmul(a, b, out=c), args.steps, args.warmup)
+            vendor[name] = {"ms": vms, "tflops": 2.0 * M * N * K / (vms * 1e-3) / 1e12}
+            log(f"4096^3 torch.matmul {name}: {vms:.4f} ms {vendor[name]['tflops']:.1f} TFLOP/s")
+            del a, b, c
+        res["vendor_torch_matmul"] = vendor
         out["main"] = res
         # the headline: the fastest split mode that stays inside the FP32 bound
         ok = [m for m in ("split3", "split6") if m in res and res[m]["errors"]["within_fp32_bound"]]
+        vendor = res.pop("vendor_torch_matmul")
         head = max(ok, key=lambda m: res[m]["tflops"]) if ok else next(iter(res))
         out["headline_mode"] = head
+        out["vendor"] = vendor
         del dA, dB, dC
         torch.cuda.empty_cache()
 
@@ -261,6 +275,7 @@ def main() -> int:
         line["plain_fp32_mfma_tflops"] = out["main"]["plain"]["tflops"]
         line["speedup_vs_plain"] = r["tflops"] / out["main"]["plain"]["tflops"]
         line["modes_4096"] = out["main"]
+        line["vendor_torch_matmul_4096"] = out["vendor"]
         line["plan_source"] = r["plan_source"]
         # Dominant kernel: the split GEMM kernel. Its algorithmic work on the
         # BF16 MFMA units is products x 2MNK per launch (one launch covers the
