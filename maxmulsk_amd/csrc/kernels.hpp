@@ -69,6 +69,15 @@ struct PackedParams {
 // into rblocks x kgroups fragments per term, zero-padded.
 hipError_t launch_pack(int mode, const float* X, long long R, long long K, long long ld_row, long long ld_k,
                        void* dst, int rblocks, int kgroups, hipStream_t stream);
+// Two operands of the same K packed by one launch.
+struct PackJob {
+    const float* X;
+    long long R, ld_row, ld_k;
+    char* dst;
+    int rblocks;
+};
+hipError_t launch_pack2(int mode, const PackJob& a, const PackJob& b, long long K, int kgroups,
+                        hipStream_t stream);
 hipError_t launch_gemm_packed(int mode, const PackedParams& p, hipStream_t stream);
 
 // C[i] = sum_{s < splits} P[s * stride + i] for i < M*N, summed in s order.

@@ -31,15 +31,23 @@ namespace mmsk {
 constexpr int kPackKT = 256;        // k values per super-tile
 constexpr int kPackPitch = kPackKT + 4;  // LDS row pitch (floats)
 
+// One launch may pack two operands of the same K (A and B of one product):
+// workgroups [0, a.rblocks) serve a, the rest serve b.
 template <int MODE>
-__global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ X, long long R, long long K,
-                                                   long long ld_row, long long ld_k, int k_major,
-                                                   char* __restrict__ dst, int rblocks, int kgroups) {
+__global__ __launch_bounds__(256) void pack_kernel(PackJob a, PackJob b, long long K, int kgroups) {
     constexpr int LU = mode_lu(MODE);  // k values per 16-byte chunk
     constexpr int KGK = 2 * LU;        // k values per fragment
     constexpr int TERMS = mode_terms(MODE);
     __shared__ __attribute__((aligned(16))) float tile[32 * kPackPitch];
-    const int rb = blockIdx.x;
+    const bool second = (int)blockIdx.x >= a.rblocks;
+    const float* __restrict__ X = second ? b.X : a.X;
+    const long long R = second ? b.R : a.R;
+    const long long ld_row = second ? b.ld_row : a.ld_row;
+    const long long ld_k = second ? b.ld_k : a.ld_k;
+    const int k_major = ld_row == 1 && ld_k != 1;
+    char* __restrict__ dst = second ? b.dst : a.dst;
+    const int rblocks = second ? b.rblocks : a.rblocks;
+    const int rb = second ? (int)blockIdx.x - a.rblocks : (int)blockIdx.x;
     const long long r0 = (long long)rb * 32;
     const long long k0 = (long long)blockIdx.y * kPackKT;
     const int tid = threadIdx.x;
@@ -315,19 +323,26 @@ int packed_block_k(int mode) {
     }
 }
 
-hipError_t launch_pack(int mode, const float* X, long long R, long long K, long long ld_row, long long ld_k,
-                       void* dst, int rblocks, int kgroups, hipStream_t stream) {
+hipError_t launch_pack2(int mode, const PackJob& a, const PackJob& b, long long K, int kgroups,
+                        hipStream_t stream) {
     const long long kpad = (long long)kgroups * packed_kgk(mode);
-    const dim3 grid((unsigned)rblocks, (unsigned)((kpad + kPackKT - 1) / kPackKT));
-    const int k_major = ld_row == 1 && ld_k != 1;
-    char* d = static_cast<char*>(dst);
+    const long long blocks = (long long)a.rblocks + b.rblocks;
+    if (blocks <= 0 || blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)blocks, (unsigned)((kpad + kPackKT - 1) / kPackKT));
     switch (mode) {
-        case 0: hipLaunchKernelGGL(pack_kernel<0>, grid, dim3(256), 0, stream, X, R, K, ld_row, ld_k, k_major, d, rblocks, kgroups); break;
-        case 1: hipLaunchKernelGGL(pack_kernel<1>, grid, dim3(256), 0, stream, X, R, K, ld_row, ld_k, k_major, d, rblocks, kgroups); break;
-        case 2: hipLaunchKernelGGL(pack_kernel<2>, grid, dim3(256), 0, stream, X, R, K, ld_row, ld_k, k_major, d, rblocks, kgroups); break;
+        case 0: hipLaunchKernelGGL(pack_kernel<0>, grid, dim3(256), 0, stream, a, b, K, kgroups); break;
+        case 1: hipLaunchKernelGGL(pack_kernel<1>, grid, dim3(256), 0, stream, a, b, K, kgroups); break;
+        case 2: hipLaunchKernelGGL(pack_kernel<2>, grid, dim3(256), 0, stream, a, b, K, kgroups); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_pack(int mode, const float* X, long long R, long long K, long long ld_row, long long ld_k,
+                       void* dst, int rblocks, int kgroups, hipStream_t stream) {
+    const PackJob a{X, R, ld_row, ld_k, static_cast<char*>(dst), rblocks};
+    const PackJob none{nullptr, 0, 1, 1, nullptr, 0};
+    return launch_pack2(mode, a, none, K, kgroups, stream);
 }
 
 hipError_t launch_gemm_packed(int mode, const PackedParams& p, hipStream_t stream) {

@@ -153,9 +153,10 @@ hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, s
         char* pb = pa + r.pack_a_bytes;
         const int terms = mode == 0 ? 1 : (mode == 1 ? 2 : 3);
         const int rbA = r.tiles_m * (r.bm / 32), rbB = r.tiles_n * (r.bn / 32);
-        e = launch_pack(mode, A, (long long)M, (long long)K, (long long)K, 1, pa, rbA, r.kgroups, stream);
-        if (e == hipSuccess)
-            e = launch_pack(mode, B, (long long)N, (long long)K, 1, (long long)N, pb, rbB, r.kgroups, stream);
+        // A rows are k-contiguous; B is read with k as the slow index.
+        const PackJob ja{A, (long long)M, (long long)K, 1, pa, rbA};
+        const PackJob jb{B, (long long)N, 1, (long long)N, pb, rbB};
+        e = launch_pack2(mode, ja, jb, (long long)K, r.kgroups, stream);
         if (e != hipSuccess) return e;
         PackedParams p{};
         p.PA = pa;
