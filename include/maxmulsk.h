@@ -255,6 +255,18 @@ maxmulsk_status maxmulsk_selector_predict(size_t M, size_t K, size_t N, maxmulsk
 
 /* ---- introspection --------------------------------------------------------- */
 
+/* Phase instrumentation. Event counters of the host runtime (allocation,
+ * stage_in, pack, compute, reduce, writeback, zero_fill), counted per process
+ * since the last reset; with MAXMULSK_ROCTX=1 the host phases are also roctx
+ * ranges ("maxmulsk/plan", "maxmulsk/allocation", "maxmulsk/stage_in",
+ * "maxmulsk/launch", "maxmulsk/writeback", "maxmulsk/sync"). Counterpart of the
+ * instrumentation scopes of sme/support/lab_hooks.hpp:10-22 and of the call
+ * counts of NB::counts (sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:448-466). */
+int maxmulsk_num_phase_counters(void);
+const char* maxmulsk_phase_counter_name(int i);
+unsigned long long maxmulsk_phase_counter(int i);
+void maxmulsk_phase_counters_reset(void);
+
 /* Number of usable gfx950 devices (0 when none). */
 int maxmulsk_device_count(void);
 /* Text of the last error on this thread (never NULL). */

@@ -101,6 +101,10 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_selector_save": (C.c_int, [C.c_char_p, _P]),
     "maxmulsk_selector_active": (C.c_int, []),
     "maxmulsk_selector_predict": (C.c_int, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(C.c_float), C.POINTER(Plan)]),
+    "maxmulsk_num_phase_counters": (C.c_int, []),
+    "maxmulsk_phase_counter_name": (C.c_char_p, [C.c_int]),
+    "maxmulsk_phase_counter": (C.c_ulonglong, [C.c_int]),
+    "maxmulsk_phase_counters_reset": (None, []),
     "maxmulsk_device_count": (C.c_int, []),
     "maxmulsk_last_error": (C.c_char_p, []),
     "maxmulsk_lds_chunk_offset": (C.c_int, [C.c_int] * 7),
@@ -133,6 +137,17 @@ def lib() -> C.CDLL:
             fn.argtypes = args
         _lib = h
     return _lib
+
+
+def phase_counters() -> dict[str, int]:
+    """Host-runtime event counts since the last reset (maxmulsk_phase_counter)."""
+    h = lib()
+    return {h.maxmulsk_phase_counter_name(i).decode(): int(h.maxmulsk_phase_counter(i))
+            for i in range(h.maxmulsk_num_phase_counters())}
+
+
+def reset_phase_counters() -> None:
+    lib().maxmulsk_phase_counters_reset()
 
 
 def last_error() -> str:

@@ -8,6 +8,7 @@
 #include "maxmulsk.h"
 #include "plan.hpp"
 #include "runtime.hpp"
+#include "trace.hpp"
 
 using namespace mmsk;
 
@@ -170,3 +171,11 @@ extern "C" int maxmulsk_tile_info(int cfg, int mode, int* out) {
     out[5] = waves;
     return 0;
 }
+
+extern "C" int maxmulsk_num_phase_counters(void) { return kNumPhaseCounters; }
+
+extern "C" const char* maxmulsk_phase_counter_name(int i) { return counter_name(i); }
+
+extern "C" unsigned long long maxmulsk_phase_counter(int i) { return (unsigned long long)counter(i); }
+
+extern "C" void maxmulsk_phase_counters_reset(void) { reset_counters(); }

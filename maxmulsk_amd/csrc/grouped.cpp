@@ -16,6 +16,7 @@
 #include "maxmulsk.h"
 #include "plan.hpp"
 #include "runtime.hpp"
+#include "trace.hpp"
 
 using namespace mmsk;
 
@@ -209,6 +210,7 @@ maxmulsk_status build(const maxmulsk_gemm* problems, size_t n, int mode, const m
     p->reduce_blocks = (int)block_problem.size();
 
     bool ok = true;
+    count(kCntAllocation);  // the plan's device buffers
     if (ws_bytes) ok = hipMalloc(&p->d_ws, ws_bytes) == hipSuccess;
     if (ok)
         for (size_t j = 0; j < gp.size(); ++j)
@@ -259,6 +261,9 @@ maxmulsk_status launch_plan(maxmulsk_grouped_plan_s* p, hipStream_t s) {
     if (p->dev < 0) return MAXMULSK_OK;
     DeviceGuard guard(p->dev);
     const hipError_t e = p->exec ? hipGraphLaunch(p->exec, s) : enqueue(p, s);
+    count(kCntZeroFill, p->zero_fills.size());
+    if (p->total_items > 0) count(kCntCompute);
+    if (p->reduce_blocks > 0) count(kCntReduce);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(MAXMULSK_ERR_DEVICE, "grouped launch failed: %s", hipGetErrorString(e));

@@ -11,6 +11,7 @@
 #include "maxmulsk.h"
 #include "plan.hpp"
 #include "runtime.hpp"
+#include "trace.hpp"
 
 using namespace mmsk;
 
@@ -60,6 +61,8 @@ extern "C" maxmulsk_status maxmulsk_pack(const float* X, size_t rows, size_t K, 
         (void)hipGetLastError();
         return fail(MAXMULSK_ERR_ALLOC, "allocation of %zu packed bytes failed", bytes);
     }
+    count(kCntAllocation);
+    count(kCntPack);
     hipStream_t s = static_cast<hipStream_t>(stream);
     // A is rows x K row-major (row stride K); B is K x rows row-major, read
     // with k as the slow index (row stride 1, k stride rows).
@@ -141,8 +144,11 @@ extern "C" maxmulsk_status maxmulsk_sgemm_packed(maxmulsk_packed A, maxmulsk_pac
         p.vec_c = (N % 4 == 0 && ((uintptr_t)C & 15) == 0);
     }
     hipError_t e = launch_gemm_packed(A->mode, p, s);
-    if (e == hipSuccess && splits > 1)
+    count(kCntCompute);
+    if (e == hipSuccess && splits > 1) {
         e = launch_splitk_reduce(partials, C, (long long)(M * N), splits, (long long)(M * N), s);
+        count(kCntReduce);
+    }
     if (e == hipSuccess && pooled) e = pool_workspace_release(pl, s);
     if (e == hipSuccess && !async) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {

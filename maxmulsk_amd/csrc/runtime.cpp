@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "trace.hpp"
 
 namespace mmsk {
 
@@ -114,10 +115,12 @@ bool grow(void** buf, size_t* cap, size_t bytes) {
         *cap = 0;
     }
     void* p = nullptr;
+    Scope scope("maxmulsk/allocation");
     if (hipMalloc(&p, want) != hipSuccess) {
         (void)hipGetLastError();
         return false;
     }
+    count(kCntAllocation);
     *buf = p;
     *cap = want;
     return true;
@@ -158,6 +161,7 @@ hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, s
         const PackJob jb{B, (long long)N, 1, (long long)N, pb, rbB};
         e = launch_pack2(mode, ja, jb, (long long)K, r.kgroups, stream);
         if (e != hipSuccess) return e;
+        count(kCntPack);
         PackedParams p{};
         p.PA = pa;
         p.PB = pb;
@@ -182,6 +186,7 @@ hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, s
             p.vec_c = (N % 4 == 0 && ((uintptr_t)C & 15) == 0);
         }
         e = launch_gemm_packed(mode, p, stream);
+        count(kCntCompute);
     } else {
         GemmParams p{};
         p.A = A;
@@ -207,8 +212,10 @@ hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, s
             p.vec_c = (N % 4 == 0 && ((uintptr_t)C & 15) == 0);
         }
         e = launch_gemm(mode, r.cfg, p, stream);
+        count(kCntCompute);
     }
     if (e != hipSuccess || r.splits == 1) return e;
+    count(kCntReduce);
     return launch_splitk_reduce(partials, C, (long long)(M * N), r.splits, (long long)(M * N), stream);
 }
 
@@ -229,6 +236,7 @@ maxmulsk_status run_gemm(const GemmCall& c) {
             return MAXMULSK_OK;
         }
         DeviceGuard g(cdev);
+        count(kCntZeroFill);
         if (launch_fill_zero_f32(c.C, c.M * c.N, c.stream) != hipSuccess ||
             (!c.async && hipStreamSynchronize(c.stream) != hipSuccess)) {
             (void)hipGetLastError();
@@ -261,14 +269,19 @@ maxmulsk_status run_gemm(const GemmCall& c) {
     if (staged && c.ws_required) return fail(MAXMULSK_ERR_UNSUPPORTED, "the workspace entry point needs device pointers");
 
     maxmulsk_plan plan = c.plan;
-    if (!c.have_plan) {
-        if (c.rule_only) {
-            if (!env_plan(&plan)) plan = rule_plan(c.M, c.K, c.N, c.mode);
-        } else {
-            plan = choose_plan(c.M, c.K, c.N, c.mode, nullptr);
+    bool resolved;
+    {
+        Scope scope("maxmulsk/plan");
+        if (!c.have_plan) {
+            if (c.rule_only) {
+                if (!env_plan(&plan)) plan = rule_plan(c.M, c.K, c.N, c.mode);
+            } else {
+                plan = choose_plan(c.M, c.K, c.N, c.mode, nullptr);
+            }
         }
+        resolved = resolve(c.M, c.K, c.N, c.mode, plan, &r);
     }
-    if (!resolve(c.M, c.K, c.N, c.mode, plan, &r))
+    if (!resolved)
         return fail(MAXMULSK_ERR_UNSUPPORTED, "plan {%d,%d} rejected for %zux%zux%zu", plan.tile,
                     plan.split_k, c.M, c.K, c.N);
 
@@ -286,6 +299,7 @@ maxmulsk_status run_gemm(const GemmCall& c) {
     const float* B = c.B;
     float* C = c.C;
     if (staged) {
+        Scope scope("maxmulsk/stage_in");
         const size_t a_bytes = c.M * c.K * sizeof(float), b_bytes = c.K * c.N * sizeof(float),
                      c_bytes = c.M * c.N * sizeof(float);
         auto up = [](size_t x) { return (x + 255) / 256 * 256; };
@@ -297,6 +311,7 @@ maxmulsk_status run_gemm(const GemmCall& c) {
                 (void)hipGetLastError();
                 return fail(MAXMULSK_ERR_DEVICE, "H2D copy of A failed");
             }
+            count(kCntStageIn);
             A = reinterpret_cast<const float*>(s);
             s += up(a_bytes);
         }
@@ -305,6 +320,7 @@ maxmulsk_status run_gemm(const GemmCall& c) {
                 (void)hipGetLastError();
                 return fail(MAXMULSK_ERR_DEVICE, "H2D copy of B failed");
             }
+            count(kCntStageIn);
             B = reinterpret_cast<const float*>(s);
             s += up(b_bytes);
         }
@@ -318,11 +334,21 @@ maxmulsk_status run_gemm(const GemmCall& c) {
         if (st != MAXMULSK_OK) return st;
     }
 
-    hipError_t e = launch_resolved(A, B, C, c.M, c.K, c.N, c.mode, r, ws, c.stream);
+    hipError_t e;
+    {
+        Scope scope("maxmulsk/launch");
+        e = launch_resolved(A, B, C, c.M, c.K, c.N, c.mode, r, ws, c.stream);
+    }
     if (e == hipSuccess && pooled) e = pool_workspace_release(pl, c.stream);
-    if (e == hipSuccess && staged && !c_dev)
+    if (e == hipSuccess && staged && !c_dev) {
+        Scope scope("maxmulsk/writeback");
         e = hipMemcpyAsync(c.C, C, c.M * c.N * sizeof(float), hipMemcpyDeviceToHost, c.stream);
-    if (e == hipSuccess && !c.async) e = hipStreamSynchronize(c.stream);
+        count(kCntWriteback);
+    }
+    if (e == hipSuccess && !c.async) {
+        Scope scope("maxmulsk/sync");
+        e = hipStreamSynchronize(c.stream);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return fail(MAXMULSK_ERR_DEVICE, "GEMM launch failed: %s", hipGetErrorString(e));

@@ -86,6 +86,15 @@ def test_pack_api_rejects_bad_arguments_without_a_device_call():
     lib.maxmulsk_packed_destroy(None)
 
 
+def test_phase_counters_names_and_reset():
+    mm.reset_phase_counters()
+    c = mm.phase_counters()
+    assert list(c) == ["allocation", "stage_in", "pack", "compute", "reduce", "writeback", "zero_fill"]
+    assert all(v == 0 for v in c.values())
+    assert mm.lib().maxmulsk_phase_counter_name(99) is None
+    assert mm.lib().maxmulsk_phase_counter(-1) == 0
+
+
 def test_invalid_plan_rejected_before_anything_runs():
     A = np.ones((8, 8), np.float32)
     C = np.full((8, 8), 9.0, np.float32)

@@ -188,3 +188,33 @@ def test_async_on_a_stream_and_k_zero():
     # host pointers cannot be asynchronous
     Ch = np.zeros((M, N), np.float32)
     assert mm.sgemm_ex(A, B, Ch, M, K, N, async_=True) == mm.ERR_UNSUPPORTED
+
+
+def test_phase_counters_follow_the_launch_structure():
+    """Launch counts per call, as NB::counts states them for the reference's
+    loop nest: direct tiles launch one GEMM (plus one reduce with split-K), the
+    256 x 256 path packs both operands in one launch first, host operands are
+    staged in and written back, and a warm pool allocates nothing."""
+    torch = torch_cuda()
+    M, N, K = 320, 384, 1024
+    A = orc.uniform(131, (M, K))
+    B = orc.uniform(132, (K, N))
+    dA, dB, dC = Guarded(M * K, A), Guarded(K * N, B), Guarded(M * N, SENTINEL)
+    torch.cuda.synchronize()
+    expect = {(64, 1): (0, 1, 0), (64, 4): (0, 1, 1), (256, 1): (1, 1, 0), (256, 2): (1, 1, 1)}
+    for plan, (packs, gemms, reduces) in expect.items():
+        assert mm.sgemm_ex(dA.ptr, dB.ptr, dC.ptr, M, K, N, mode=1, plan=plan) == mm.OK  # warms the pool
+        mm.reset_phase_counters()
+        assert mm.sgemm_ex(dA.ptr, dB.ptr, dC.ptr, M, K, N, mode=1, plan=plan) == mm.OK
+        c = mm.phase_counters()
+        assert (c["pack"], c["compute"], c["reduce"]) == (packs, gemms, reduces), (plan, c)
+        assert c["allocation"] == 0 and c["stage_in"] == 0 and c["writeback"] == 0, (plan, c)
+    Ch = np.full((M, N), SENTINEL, np.float32)
+    assert mm.sgemm_v4(A, B, Ch, M, K, N) == mm.OK
+    mm.reset_phase_counters()
+    assert mm.sgemm_v4(A, B, Ch, M, K, N) == mm.OK
+    c = mm.phase_counters()
+    assert c["stage_in"] == 2 and c["writeback"] == 1 and c["compute"] == 1 and c["allocation"] == 0, c
+    mm.reset_phase_counters()
+    assert mm.sgemm_v4(dA.ptr, dB.ptr, dC.ptr, M, 0, N) == mm.OK
+    assert mm.phase_counters()["zero_fill"] == 1
