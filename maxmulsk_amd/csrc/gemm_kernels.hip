@@ -20,7 +20,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmParams p) {
     const long long kend = min(p.K, kbeg + p.k_len);
     T t;
     t.run(p.A, p.B, p.C + split * p.c_split_stride, p.M, p.N, p.lda, p.ldb, p.ldc,
-          (long long)tm * BM, (long long)tn * BN, kbeg, kend, p.vec_ab != 0, p.vec_c != 0, smem);
+          (long long)tm * BM, (long long)tn * BN, kbeg, kend, smem);
 }
 
 // Sums the split-K partials in slice order (deterministic).
@@ -29,14 +29,13 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const float* __restr
                                                             int splits, long long stride) {
     const long long step = (long long)gridDim.x * 256 * 4;
     for (long long i = ((long long)blockIdx.x * 256 + threadIdx.x) * 4; i < mn; i += step) {
-        if (i + 3 < mn && (stride & 3) == 0 && ((uintptr_t)(C + i) & 15) == 0 &&
-            ((uintptr_t)(P + i) & 15) == 0) {
-            f32x4 s = *reinterpret_cast<const f32x4*>(P + i);
+        if (i + 3 < mn) {
+            f32x4 s = *reinterpret_cast<const f32x4u*>(P + i);
             for (int k = 1; k < splits; ++k) {
-                const f32x4 v = *reinterpret_cast<const f32x4*>(P + k * stride + i);
+                const f32x4 v = *reinterpret_cast<const f32x4u*>(P + k * stride + i);
                 s += v;
             }
-            *reinterpret_cast<f32x4*>(C + i) = s;
+            *reinterpret_cast<f32x4u*>(C + i) = s;
         } else {
             for (long long j = i; j < i + 4 && j < mn; ++j) {
                 float s = P[j];

@@ -31,6 +31,10 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// Global-memory vectors that only promise dword alignment: rows of A, B and C
+// start at any float when K or N is not a multiple of 4. Multi-dword global
+// loads and stores need only dword alignment on gfx950.
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 
 // Two FP32 values -> two BF16 values (round to nearest even), packed with the
 // first value in the low half (v_cvt_pk_bf16_f32).
@@ -124,65 +128,56 @@ struct Tile {
 
     f32x4 ra[UA][LU / 4];  // A unit i: LU consecutive k of one row
     using bvec = float __attribute__((ext_vector_type(BW)));
+    using bvecu = float __attribute__((ext_vector_type(BW), aligned(4)));
     bvec rb[UB][LU];       // B unit i: row e holds BW consecutive columns
     f32x16 acc[TM][TN];
 
+    // Global -> registers for the k-block at k0. fast: the whole block is inside
+    // M, N and [k0, kend), so every unit is a plain vector load. Otherwise each
+    // unit is a vector load when it is inside, zero when it is wholly outside,
+    // and element-wise only when it straddles an edge.
     __device__ __forceinline__ void load(const float* __restrict__ A, const float* __restrict__ B,
                                          long long M, long long N, long long lda, long long ldb,
                                          long long m0, long long n0, long long k0, long long kend,
                                          bool fast) {
         const int tid = threadIdx.x;
-        if (fast) {
 #pragma unroll
-            for (int i = 0; i < UA; ++i) {
-                const int u = tid + NT * i;
-                if (A_UNITS % NT == 0 || u < A_UNITS) {
-                    const int row = u / CPR, c = u % CPR;
-                    const float* src = A + (m0 + row) * lda + k0 + c * LU;
+        for (int i = 0; i < UA; ++i) {
+            const int u = tid + NT * i;
+            if (A_UNITS % NT == 0 || u < A_UNITS) {
+                const int row = u / CPR, c = u % CPR;
+                const long long gm = m0 + row, gk = k0 + c * LU;
+                const float* src = A + gm * lda + gk;
+                if (fast || (gm < M && gk + LU <= kend)) {
 #pragma unroll
-                    for (int q = 0; q < LU / 4; ++q)
-                        ra[i][q] = *reinterpret_cast<const f32x4*>(src + 4 * q);
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < UB; ++i) {
-                const int u = tid + NT * i;
-                if (B_UNITS % NT == 0 || u < B_UNITS) {
-                    const int nb = u % (BN / BW), c = u / (BN / BW);
-                    const float* src = B + (k0 + c * LU) * ldb + n0 + BW * nb;
-#pragma unroll
-                    for (int e = 0; e < LU; ++e)
-                        rb[i][e] = *reinterpret_cast<const bvec*>(src + e * ldb);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < UA; ++i) {
-                const int u = tid + NT * i;
-                if (A_UNITS % NT == 0 || u < A_UNITS) {
-                    const int row = u / CPR, c = u % CPR;
-                    const long long gm = m0 + row, gk = k0 + c * LU;
+                    for (int q = 0; q < LU / 4; ++q) ra[i][q] = *reinterpret_cast<const f32x4u*>(src + 4 * q);
+                } else {
 #pragma unroll
                     for (int e = 0; e < LU; ++e) {
                         float x = 0.f;
-                        if (gm < M && gk + e < kend) x = A[gm * lda + gk + e];
+                        if (gm < M && gk + e < kend) x = src[e];
                         ra[i][e / 4][e % 4] = x;
                     }
                 }
             }
+        }
 #pragma unroll
-            for (int i = 0; i < UB; ++i) {
-                const int u = tid + NT * i;
-                if (B_UNITS % NT == 0 || u < B_UNITS) {
-                    const int nb = u % (BN / BW), c = u / (BN / BW);
-                    const long long gn = n0 + BW * nb;
+        for (int i = 0; i < UB; ++i) {
+            const int u = tid + NT * i;
+            if (B_UNITS % NT == 0 || u < B_UNITS) {
+                const int nb = u % (BN / BW), c = u / (BN / BW);
+                const long long gn = n0 + BW * nb, gk = k0 + c * LU;
+                const float* src = B + gk * ldb + gn;
+                if (fast || (gn + BW <= N && gk + LU <= kend)) {
+#pragma unroll
+                    for (int e = 0; e < LU; ++e) rb[i][e] = *reinterpret_cast<const bvecu*>(src + e * ldb);
+                } else {
 #pragma unroll
                     for (int e = 0; e < LU; ++e) {
-                        const long long gk = k0 + c * LU + e;
 #pragma unroll
                         for (int j = 0; j < BW; ++j) {
                             float x = 0.f;
-                            if (gk < kend && gn + j < N) x = B[gk * ldb + gn + j];
+                            if (gk + e < kend && gn + j < N) x = src[e * ldb + j];
                             rb[i][e][j] = x;
                         }
                     }
@@ -318,10 +313,9 @@ struct Tile {
 
     // Accumulator (lane l, register 4q + e) of block (ti, tj) holds
     // C[m0 + wm*WTM + ti*32 + l%32][n0 + wn*WTN + tj*32 + 8q + 4(l/32) + e]:
-    // four consecutive columns per q, stored as one dwordx4 where aligned.
+    // four consecutive columns per q, stored as one dwordx4 inside N.
     __device__ __forceinline__ void epilogue(float* __restrict__ C, long long M, long long N,
-                                             long long ldc, long long m0, long long n0,
-                                             bool vec_c) const {
+                                             long long ldc, long long m0, long long n0) const {
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         const int wm = wave / WGN, wn = wave % WGN;
         const int r = lane & 31, h = lane >> 5;
@@ -337,8 +331,8 @@ struct Tile {
                     const long long n = n0 + wn * WTN + tj * 32 + 8 * q + 4 * h;
                     const f32x4 v = {acc[ti][tj][4 * q + 0], acc[ti][tj][4 * q + 1],
                                      acc[ti][tj][4 * q + 2], acc[ti][tj][4 * q + 3]};
-                    if (vec_c && n + 3 < N) {
-                        *reinterpret_cast<f32x4*>(crow + n) = v;
+                    if (n + 3 < N) {
+                        *reinterpret_cast<f32x4u*>(crow + n) = v;
                     } else {
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
@@ -350,7 +344,7 @@ struct Tile {
     }
 
     // Main loop over the k-blocks of [kbeg, kend). ALL_FAST: every block is a
-    // full, aligned block (dwordx4 loads, no bounds checks), so the steady-state
+    // full block (vector loads, no bounds checks), so the steady-state
     // body is one basic block and the scheduler can interleave the MFMAs of
     // stage kb with the split/store of stage kb+1. Registers for block kb+2 are
     // loaded once the store of kb+1 has consumed them, i.e. one full iteration
@@ -385,7 +379,7 @@ struct Tile {
                                         float* __restrict__ C, long long M, long long N,
                                         long long lda, long long ldb, long long ldc,
                                         long long m0, long long n0, long long kbeg, long long kend,
-                                        bool vec_ab, bool vec_c, char* smem) {
+                                        char* smem) {
 #pragma unroll
         for (int ti = 0; ti < TM; ++ti)
 #pragma unroll
@@ -393,12 +387,12 @@ struct Tile {
 #pragma unroll
                 for (int e = 0; e < 16; ++e) acc[ti][tj][e] = 0.f;
 
-        const bool full_mn = (m0 + BM <= M) && (n0 + BN <= N) && vec_ab;
+        const bool full_mn = (m0 + BM <= M) && (n0 + BN <= N);
         if (full_mn && (kend - kbeg) % BK == 0)
             mainloop<true>(A, B, M, N, lda, ldb, m0, n0, kbeg, kend, full_mn, smem);
         else
             mainloop<false>(A, B, M, N, lda, ldb, m0, n0, kbeg, kend, full_mn, smem);
-        epilogue(C, M, N, ldc, m0, n0, vec_c);
+        epilogue(C, M, N, ldc, m0, n0);
     }
 };
 

@@ -185,7 +185,6 @@ maxmulsk_status build(const maxmulsk_gemm* problems, size_t n, int mode, const m
         q.cfg = r.cfg;
         q.tiles_m = r.tiles_m;
         q.tiles_n = r.tiles_n;
-        q.vec_ab = (g.K % 4 == 0 && g.N % 4 == 0 && ((uintptr_t)g.A & 15) == 0 && ((uintptr_t)g.B & 15) == 0);
         q.item_begin = (int)items;
         const long long cnt = (long long)r.tiles_m * r.tiles_n * r.splits;
         items += cnt;
@@ -197,12 +196,10 @@ maxmulsk_status build(const maxmulsk_gemm* problems, size_t n, int mode, const m
         if (r.splits > 1) {
             ws_off[j] = ws_bytes;
             ws_bytes += (r.ws_bytes + 255) / 256 * 256;
-            q.vec_c = (g.N % 4 == 0);
             q.reduce_begin = (int)block_problem.size();
             const long long blocks = ((long long)g.M * g.N + 1023) / 1024;
             for (long long b = 0; b < blocks; ++b) block_problem.push_back((int)j);
         } else {
-            q.vec_c = (g.N % 4 == 0 && ((uintptr_t)g.C & 15) == 0);
             q.reduce_begin = 0;
         }
     }

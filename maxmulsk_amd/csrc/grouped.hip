@@ -45,7 +45,7 @@ __device__ __forceinline__ void run_item(const GroupProblem& p, int local, char*
     float* dst = p.splits > 1 ? p.P + split * p.M * p.N : p.C;
     CfgTile<MODE, CFG> t;
     t.run(p.A, p.B, dst, p.M, p.N, p.K, p.N, p.N, (long long)tm * c.bm, (long long)tn * c.bn, kbeg,
-          kend, p.vec_ab != 0, p.vec_c != 0, smem);
+          kend, smem);
 }
 
 template <int MODE>
@@ -75,11 +75,10 @@ __global__ __launch_bounds__(256) void grouped_reduce_kernel(const GroupProblem*
     const GroupProblem& p = problems[block_problem[blockIdx.x]];
     const long long mn = p.M * p.N;
     const long long base = (long long)(blockIdx.x - p.reduce_begin) * 1024 + threadIdx.x * 4;
-    const bool vec = (mn & 3) == 0 && ((uintptr_t)p.C & 15) == 0;
-    if (vec && base + 3 < mn) {
-        f32x4 s = *reinterpret_cast<const f32x4*>(p.P + base);
-        for (int k = 1; k < p.splits; ++k) s += *reinterpret_cast<const f32x4*>(p.P + k * mn + base);
-        *reinterpret_cast<f32x4*>(p.C + base) = s;
+    if (base + 3 < mn) {
+        f32x4 s = *reinterpret_cast<const f32x4u*>(p.P + base);
+        for (int k = 1; k < p.splits; ++k) s += *reinterpret_cast<const f32x4u*>(p.P + k * mn + base);
+        *reinterpret_cast<f32x4u*>(p.C + base) = s;
     } else {
         for (long long j = base; j < base + 4 && j < mn; ++j) {
             float s = p.P[j];

@@ -40,8 +40,6 @@ struct GemmParams {
     long long c_split_stride;
     int splits;
     int tiles_m, tiles_n;
-    int vec_ab;  // A and B rows 16-byte aligned (K % 4 == 0, N % 4 == 0, aligned bases)
-    int vec_c;   // C rows 16-byte aligned
 };
 
 hipError_t launch_gemm(int mode, int cfg, const GemmParams& p, hipStream_t stream);
@@ -63,7 +61,6 @@ struct PackedParams {
     long long kblocks;            // k-blocks of kPackedBK in the packed operands
     long long kblocks_per_split;
     int splits, tiles_m, tiles_n;
-    int vec_c;
 };
 // Packs X (element (row, k) at X[row * ld_row + k * ld_k], rows < R, k < K)
 // into rblocks x kgroups fragments per term, zero-padded.
@@ -97,7 +94,6 @@ struct GroupProblem {
     int splits;
     int cfg;       // tile geometry index into kTileCfgs
     int tiles_m, tiles_n;
-    int vec_ab, vec_c;
     int item_begin;   // first work item of this problem in the flattened list
     int reduce_begin; // first reduce block of this problem (split problems only)
 };

@@ -61,8 +61,8 @@ __global__ __launch_bounds__(256) void pack_kernel(PackJob a, PackJob b, long lo
             f32x4 v = {0.f, 0.f, 0.f, 0.f};
             if (gr < R) {
                 const float* src = X + gr * ld_row + gk * ld_k;
-                if (gk + 3 < K && ld_k == 1 && (((uintptr_t)src) & 15) == 0) {
-                    v = *reinterpret_cast<const f32x4*>(src);
+                if (gk + 3 < K && ld_k == 1) {
+                    v = *reinterpret_cast<const f32x4u*>(src);
                 } else {
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
@@ -81,8 +81,8 @@ __global__ __launch_bounds__(256) void pack_kernel(PackJob a, PackJob b, long lo
             f32x4 v = {0.f, 0.f, 0.f, 0.f};
             if (gk < K) {
                 const float* src = X + gk * ld_k + gr * ld_row;
-                if (gr + 3 < R && ld_row == 1 && (((uintptr_t)src) & 15) == 0) {
-                    v = *reinterpret_cast<const f32x4*>(src);
+                if (gr + 3 < R && ld_row == 1) {
+                    v = *reinterpret_cast<const f32x4u*>(src);
                 } else {
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
@@ -234,7 +234,7 @@ struct PackedTile {
     __device__ __forceinline__ void run(const char* __restrict__ PA, const char* __restrict__ PB, long long planeA,
                                         long long planeB, int kgtA, int kgtB, int tm, int tn, long long kb0,
                                         long long nkb, float* __restrict__ C, long long M, long long N, long long ldc,
-                                        bool vec_c, char* smem) {
+                                        char* smem) {
 #pragma unroll
         for (int ti = 0; ti < TM; ++ti)
 #pragma unroll
@@ -276,8 +276,8 @@ struct PackedTile {
                     const long long n = n0 + wn * WTN + tj * 32 + 8 * q + 4 * h;
                     const f32x4 v = {acc[ti][tj][4 * q + 0], acc[ti][tj][4 * q + 1], acc[ti][tj][4 * q + 2],
                                      acc[ti][tj][4 * q + 3]};
-                    if (vec_c && n + 3 < N) {
-                        *reinterpret_cast<f32x4*>(crow + n) = v;
+                    if (n + 3 < N) {
+                        *reinterpret_cast<f32x4u*>(crow + n) = v;
                     } else {
 #pragma unroll
                         for (int e = 0; e < 4; ++e)
@@ -312,7 +312,7 @@ __global__ __launch_bounds__(PackedCfg<MODE>::T::NT) void gemm_packed_kernel(Pac
     if (nkb > p.kblocks_per_split) nkb = p.kblocks_per_split;
     T t;
     t.run(p.PA, p.PB, p.planeA, p.planeB, p.kgtA, p.kgtB, tm, tn, kb0, nkb, p.C + split * p.c_split_stride, p.M, p.N,
-          p.ldc, p.vec_c != 0, smem);
+          p.ldc, smem);
 }
 
 int packed_block_k(int mode) {

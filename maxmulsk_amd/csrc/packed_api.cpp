@@ -137,11 +137,9 @@ extern "C" maxmulsk_status maxmulsk_sgemm_packed(maxmulsk_packed A, maxmulsk_pac
     if (splits > 1) {
         p.C = partials;
         p.c_split_stride = (long long)(M * N);
-        p.vec_c = (N % 4 == 0);
     } else {
         p.C = C;
         p.c_split_stride = 0;
-        p.vec_c = (N % 4 == 0 && ((uintptr_t)C & 15) == 0);
     }
     hipError_t e = launch_gemm_packed(A->mode, p, s);
     count(kCntCompute);

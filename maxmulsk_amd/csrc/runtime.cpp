@@ -179,11 +179,9 @@ hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, s
         if (r.splits > 1) {
             p.C = partials;
             p.c_split_stride = (long long)(M * N);
-            p.vec_c = (N % 4 == 0);
         } else {
             p.C = C;
             p.c_split_stride = 0;
-            p.vec_c = (N % 4 == 0 && ((uintptr_t)C & 15) == 0);
         }
         e = launch_gemm_packed(mode, p, stream);
         count(kCntCompute);
@@ -201,15 +199,12 @@ hipError_t launch_resolved(const float* A, const float* B, float* C, size_t M, s
         p.splits = r.splits;
         p.tiles_m = r.tiles_m;
         p.tiles_n = r.tiles_n;
-        p.vec_ab = (K % 4 == 0 && N % 4 == 0 && ((uintptr_t)A & 15) == 0 && ((uintptr_t)B & 15) == 0);
         if (r.splits > 1) {
             p.C = partials;
             p.c_split_stride = (long long)(M * N);
-            p.vec_c = (N % 4 == 0);
         } else {
             p.C = C;
             p.c_split_stride = 0;
-            p.vec_c = (N % 4 == 0 && ((uintptr_t)C & 15) == 0);
         }
         e = launch_gemm(mode, r.cfg, p, stream);
         count(kCntCompute);
