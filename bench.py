@@ -397,20 +397,31 @@ def cpu_baseline(seconds: float) -> dict:
         blas = f"{info.get('name')} {info.get('version')}"
     except Exception:  # pragma: no cover - informational only
         blas = "numpy BLAS"
-    blas_rows = {}
-    for m in (256, 4096):
+    # spread of the portable samples: (p90 - p10) / median, in percent
+    res["spread_pct_p10_p90"] = 100.0 * (float(np.quantile(samples, 0.9)) - float(np.quantile(samples, 0.1))) / med
+
+    def blas_rate(m: int, seconds_each: float) -> float:
         a = orc.uniform(5, (m, m))
         b = orc.uniform(6, (m, m))
         a @ b
         ts = []
         t0 = time.perf_counter()
-        while time.perf_counter() - t0 < min(3.0, seconds) or len(ts) < 3:
+        while time.perf_counter() - t0 < seconds_each or len(ts) < 3:
             t = time.perf_counter()
             a @ b
             ts.append(time.perf_counter() - t)
-        blas_rows[f"{m}^3"] = 2.0 * m ** 3 / float(np.median(ts)) / 1e12
-    res["host_blas_sgemm_tflops"] = blas_rows
-    res["host_blas"] = f"{blas}, all {os.cpu_count()} threads"
+        return 2.0 * m ** 3 / float(np.median(ts)) / 1e12
+
+    each = min(3.0, seconds)
+    rows = {"all_threads": {f"{m}^3": blas_rate(m, each) for m in (256, 4096)}}
+    try:
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=1, user_api="blas"):
+            rows["1_thread"] = {f"{m}^3": blas_rate(m, each) for m in (256, 1024)}
+    except ImportError:  # pragma: no cover - informational only
+        pass
+    res["host_blas_sgemm_tflops"] = rows
+    res["host_blas"] = f"{blas}; all_threads = {os.cpu_count()} logical CPUs visible"
     return res
 
 
