@@ -167,7 +167,7 @@ extern "C" int maxmulsk_tile_info(int cfg, int mode, int* out) {
     out[1] = t.bn;
     out[2] = t.bk[mode];
     out[3] = 64 * waves;
-    out[4] = waves == 8 ? 2 : 4;  // B staging unit width used by the tile kernel
+    out[4] = b_unit_cols(mode, t.bn, t.bk[mode], waves);  // B staging unit width of the tile kernel
     out[5] = waves;
     return 0;
 }

@@ -107,9 +107,8 @@ struct Tile {
     static constexpr int CPR = BK / LU;             // 16-byte chunks per row of a k-block
     static constexpr int KG = CPR / 2;              // 2-chunk k-groups (one MFMA fragment deep)
     static constexpr int A_UNITS = BM * CPR;        // A unit: one chunk of one row
-    // B unit: LU k-rows x BW columns; eight-wave tiles use 2 columns to keep the
-    // staging registers within the two-waves-per-SIMD budget.
-    static constexpr int BW = (WGM * WGN == 8) ? 2 : 4;
+    // B unit: LU k-rows x BW columns.
+    static constexpr int BW = b_unit_cols(MODE, BN, BK, WGM * WGN);
     static constexpr int B_UNITS = CPR * (BN / BW);
     static constexpr int NT = 64 * WGM * WGN;       // threads per workgroup
     static constexpr int UA = (A_UNITS + NT - 1) / NT;

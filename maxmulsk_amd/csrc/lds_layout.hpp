@@ -64,6 +64,17 @@ MMSK_HD constexpr int chunk_offset(int mode, int op, int rows, int bk, int row, 
     return ((term * nfrag + frag) * 64 + swizzle_slot(op, r, rblk, c, cpr)) * 16;
 }
 
+// Columns per B staging unit of the tile kernel (a unit is LU k-rows of BW
+// consecutive columns). Eight-wave tiles use 2 to keep the staging registers
+// within the two-waves-per-SIMD budget. Plain four-wave tiles use 4 (measured
+// faster than narrower units). Split four-wave tiles use as few columns as
+// still give every thread a unit, so narrow tiles stage B with all threads and
+// few registers (measured faster).
+MMSK_HD constexpr int b_unit_cols(int mode, int bn, int bk, int waves) {
+    const int per_thread = (bk / mode_lu(mode)) * bn / (64 * waves);
+    return waves == 8 ? 2 : (mode == 0 || per_thread >= 4 ? 4 : (per_thread >= 2 ? 2 : 1));
+}
+
 // Bytes of one operand region (all terms) of one stage.
 MMSK_HD constexpr int operand_bytes(int mode, int rows, int bk) {
     return mode_terms(mode) * rows * bk * (mode == 0 ? 4 : 2);
