@@ -315,14 +315,6 @@ __global__ __launch_bounds__(PackedCfg<MODE>::T::NT) void gemm_packed_kernel(Pac
           p.ldc, smem);
 }
 
-int packed_block_k(int mode) {
-    switch (mode) {
-        case 0: return PackedCfg<0>::T::KGK * PackedCfg<0>::T::KGB;
-        case 1: return PackedCfg<1>::T::KGK * PackedCfg<1>::T::KGB;
-        default: return PackedCfg<2>::T::KGK * PackedCfg<2>::T::KGB;
-    }
-}
-
 hipError_t launch_pack2(int mode, const PackJob& a, const PackJob& b, long long K, int kgroups,
                         hipStream_t stream) {
     const long long kpad = (long long)kgroups * packed_kgk(mode);
