@@ -7,9 +7,10 @@
 
 namespace mmsk {
 
-// Tile geometries compiled for every mode: wgm x wgn waves over the output
-// tile. The 256 x 256 geometry runs eight waves (two per SIMD) so one wave's
-// staging work overlaps the other's MFMAs; the others run four.
+// Tile geometries: wgm x wgn waves over the output tile. Geometries 0..4 are
+// compiled for every mode as the direct tile kernel (and serve the grouped
+// kernel); geometry 5 (256 x 256, eight waves, two per SIMD) runs on the
+// packed path, whose k-block depth is kPackedBK.
 struct TileCfg {
     int bm, bn;
     int wgm, wgn;
