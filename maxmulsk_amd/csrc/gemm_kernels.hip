@@ -4,6 +4,7 @@
 
 #include "gemm_tile.hpp"
 #include "kernels.hpp"
+#include "splitk_sum.hpp"
 
 namespace mmsk {
 
@@ -30,12 +31,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const float* __restr
     const long long step = (long long)gridDim.x * 256 * 4;
     for (long long i = ((long long)blockIdx.x * 256 + threadIdx.x) * 4; i < mn; i += step) {
         if (i + 3 < mn) {
-            f32x4 s = *reinterpret_cast<const f32x4u*>(P + i);
-            for (int k = 1; k < splits; ++k) {
-                const f32x4 v = *reinterpret_cast<const f32x4u*>(P + k * stride + i);
-                s += v;
-            }
-            *reinterpret_cast<f32x4u*>(C + i) = s;
+            *reinterpret_cast<f32x4u*>(C + i) = sum_splits4(P + i, stride, splits);
         } else {
             for (long long j = i; j < i + 4 && j < mn; ++j) {
                 float s = P[j];

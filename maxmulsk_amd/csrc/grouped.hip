@@ -12,6 +12,7 @@
 
 #include "gemm_tile.hpp"
 #include "kernels.hpp"
+#include "splitk_sum.hpp"
 
 namespace mmsk {
 
@@ -76,9 +77,7 @@ __global__ __launch_bounds__(256) void grouped_reduce_kernel(const GroupProblem*
     const long long mn = p.M * p.N;
     const long long base = (long long)(blockIdx.x - p.reduce_begin) * 1024 + threadIdx.x * 4;
     if (base + 3 < mn) {
-        f32x4 s = *reinterpret_cast<const f32x4u*>(p.P + base);
-        for (int k = 1; k < p.splits; ++k) s += *reinterpret_cast<const f32x4u*>(p.P + k * mn + base);
-        *reinterpret_cast<f32x4u*>(p.C + base) = s;
+        *reinterpret_cast<f32x4u*>(p.C + base) = sum_splits4(p.P + base, mn, p.splits);
     } else {
         for (long long j = base; j < base + 4 && j < mn; ++j) {
             float s = p.P[j];

@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import csv
+import ctypes
 import sys
 import time
 
@@ -61,13 +62,17 @@ def main(argv=None) -> int:
     def make_call(name, dA, dB, dC, M, K, N):
         if name == "vendor":
             return lambda: torch.matmul(dA, dB, out=dC)
-        mode = mm.MODES[name]
-        a, b, c = dA.data_ptr(), dB.data_ptr(), dC.data_ptr()
+        # arguments built once, as a C caller holds them
+        opt = mm.Options(mode=mm.MODES[name], plan=mm.Plan(0, 0), stream=sptr, async_=1,
+                         workspace=None, workspace_bytes=0)
+        fn = mm.lib().maxmulsk_sgemm_ex
+        args_ = (dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, ctypes.byref(opt))
 
         def call():
-            st = mm.sgemm_ex(a, b, c, M, K, N, mode=mode, stream=sptr, async_=True)
+            st = fn(*args_)
             if st != mm.OK:
                 raise RuntimeError(f"{name} status {st}: {mm.last_error()}")
+        call.opt = opt
         return call
 
     def time_calls(fn, reps):
