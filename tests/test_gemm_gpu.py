@@ -28,6 +28,30 @@ def test_small_integers_bit_exact_every_plan(mode, shape):
         assert bad.size == 0, f"mode {mode} plan {plan}: {len(bad)} wrong, first {bad[:4].tolist()}"
 
 
+def _up256(x):
+    return (x + 255) // 256 * 256
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_every_split_count_reduces_exactly(mode):
+    """Plan (tile, 16) on a K of s k-blocks runs exactly s slices, so the reduce
+    sees every split count 1..16 (direct 64-row tiles: k-blocks of 32; packed
+    256 x 256 tiles: k-blocks of 16). Small integers keep every partial sum
+    exact, so the result must equal the integer product bit for bit."""
+    for s in range(1, 17):
+        for M, N, tile, depth in ((67, 93, 64, 32), (300, 260, 256, 16)):
+            K = depth * s
+            extra = mm.workspace_bytes(M, K, N, mode, (tile, 16)) - (
+                mm.workspace_bytes(M, K, N, mode, (tile, 1)))
+            assert extra == (_up256(s * M * N * 4) if s > 1 else 0), (s, tile, extra)
+            A = orc.small_ints(3000 + s, (M, K), 8)
+            B = orc.small_ints(4000 + s, (K, N), 8)
+            exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float32)
+            Cg = gpu_gemm(A, B, mode=mode, plan=(tile, 16))
+            bad = np.argwhere(Cg != exact)
+            assert bad.size == 0, f"mode {mode} splits {s} tile {tile}: {len(bad)} wrong, first {bad[:4].tolist()}"
+
+
 # Plain FP32 MFMA on the measured shapes (M x N x K), checked against FP64 within
 # |C - C64| <= K * 2^-24 * |A||B| (gamma = 1).
 PLAIN_SHAPES = [

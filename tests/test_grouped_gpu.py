@@ -63,6 +63,25 @@ def test_grouped_matches_oracle_and_standalone(mode):
             assert np.array_equal(ref.numpy((M, N)), Cg), (M, K, N, plan)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_grouped_reduce_every_split_count(mode):
+    """Sixteen problems whose K spans 1..16 k-blocks of 32 under plan (64, 16):
+    one grouped reduce serves every split count. Small-integer inputs make the
+    products exact."""
+    torch = torch_cuda()
+    specs = [(40 + s, 32 * s, 70 + 3 * s) for s in range(1, 17)]
+    probs, host = build(specs, 800, gen=lambda seed, shape: orc.small_ints(seed, shape, 8))
+    gp = mm.GroupedPlan(as_args(probs), mode=mode, plans=[(64, 16)] * len(specs))
+    gp.launch()
+    torch.cuda.synchronize()
+    gp.close()
+    for (dA, dB, dC, M, K, N), (A, B) in zip(probs, host):
+        exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float32)
+        got = dC.numpy((M, N))
+        assert np.array_equal(got, exact), f"mode {mode} K {K}: {int((got != exact).sum())} wrong"
+        assert dC.guards_intact()
+
+
 def test_graph_replays_are_identical():
     torch = torch_cuda()
     probs, host = build(SPECS, 600)
