@@ -134,3 +134,28 @@ def test_concurrent_callers_on_several_streams():
         t.join(timeout=120)
     assert not any(t.is_alive() for t in threads)
     assert not errors, errors
+
+
+def test_async_calls_alternating_streams_share_the_pool():
+    """Asynchronous split-K calls alternate between two streams with no host
+    synchronisation in between; all of them use the library's one pooled
+    workspace. Each call must wait for the previous user of the pool on the
+    other stream, or partials would be overwritten while still being reduced."""
+    torch = torch_cuda()
+    M, N, K = 128, 1024, 8192
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    calls = []
+    for i in range(8):
+        A = orc.uniform(900 + 2 * i, (M, K))
+        B = orc.uniform(901 + 2 * i, (K, N))
+        dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+        dC = torch.full((M, N), SENTINEL, dtype=torch.float32, device="cuda")
+        calls.append((A, B, dA, dB, dC))
+    torch.cuda.synchronize()
+    for i, (A, B, dA, dB, dC) in enumerate(calls):
+        st = mm.sgemm_ex(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, mode=1 + i % 2,
+                         plan=(64, 16), stream=streams[i % 2], async_=True)
+        assert st == mm.OK, mm.last_error()
+    torch.cuda.synchronize()
+    for i, (A, B, dA, dB, dC) in enumerate(calls):
+        assert full_check(A, B, dC.cpu().numpy())["bound_ratio"] <= 1.0, i
