@@ -100,7 +100,7 @@ __host__ __device__ constexpr int product_term(int mode, int p, int side) {
                                   : (p == 1 ? 2 : (p == 2 || p == 4) ? 1 : 0));
 }
 
-template <int MODE, int BM, int BN, int BK, int WGM, int WGN>
+template <int MODE, int BM, int BN, int BK, int WGM, int WGN, int PF_ = prefetch_sets(MODE, BM, BN)>
 struct Tile {
     static constexpr int LU = mode_lu(MODE);
     static constexpr int TERMS = mode_terms(MODE);
@@ -125,16 +125,20 @@ struct Tile {
     static_assert(TM >= 1 && TN >= 1, "wave tile must hold a 32x32 block");
     static_assert(SMEM <= 160 * 1024, "LDS budget of one CU");
 
-    f32x4 ra[UA][LU / 4];  // A unit i: LU consecutive k of one row
+    // Register sets for k-blocks in flight between global memory and LDS.
+    static constexpr int PF = PF_;
+    static_assert(PF == 1 || PF == 2, "one or two register sets");
+    f32x4 ra[PF][UA][LU / 4];  // A unit i: LU consecutive k of one row
     using bvec = float __attribute__((ext_vector_type(BW)));
     using bvecu = float __attribute__((ext_vector_type(BW), aligned(4)));
-    bvec rb[UB][LU];       // B unit i: row e holds BW consecutive columns
+    bvec rb[PF][UB][LU];       // B unit i: row e holds BW consecutive columns
     f32x16 acc[TM][TN];
 
     // Global -> registers for the k-block at k0. fast: the whole block is inside
     // M, N and [k0, kend), so every unit is a plain vector load. Otherwise each
     // unit is a vector load when it is inside, zero when it is wholly outside,
     // and element-wise only when it straddles an edge.
+    template <int S>
     __device__ __forceinline__ void load(const float* __restrict__ A, const float* __restrict__ B,
                                          long long M, long long N, long long lda, long long ldb,
                                          long long m0, long long n0, long long k0, long long kend,
@@ -149,13 +153,13 @@ struct Tile {
                 const float* src = A + gm * lda + gk;
                 if (fast || (gm < M && gk + LU <= kend)) {
 #pragma unroll
-                    for (int q = 0; q < LU / 4; ++q) ra[i][q] = *reinterpret_cast<const f32x4u*>(src + 4 * q);
+                    for (int q = 0; q < LU / 4; ++q) ra[S][i][q] = *reinterpret_cast<const f32x4u*>(src + 4 * q);
                 } else {
 #pragma unroll
                     for (int e = 0; e < LU; ++e) {
                         float x = 0.f;
                         if (gm < M && gk + e < kend) x = src[e];
-                        ra[i][e / 4][e % 4] = x;
+                        ra[S][i][e / 4][e % 4] = x;
                     }
                 }
             }
@@ -169,7 +173,7 @@ struct Tile {
                 const float* src = B + gk * ldb + gn;
                 if (fast || (gn + BW <= N && gk + LU <= kend)) {
 #pragma unroll
-                    for (int e = 0; e < LU; ++e) rb[i][e] = *reinterpret_cast<const bvecu*>(src + e * ldb);
+                    for (int e = 0; e < LU; ++e) rb[S][i][e] = *reinterpret_cast<const bvecu*>(src + e * ldb);
                 } else {
 #pragma unroll
                     for (int e = 0; e < LU; ++e) {
@@ -177,7 +181,7 @@ struct Tile {
                         for (int j = 0; j < BW; ++j) {
                             float x = 0.f;
                             if (gk + e < kend && gn + j < N) x = src[e * ldb + j];
-                            rb[i][e][j] = x;
+                            rb[S][i][e][j] = x;
                         }
                     }
                 }
@@ -185,6 +189,7 @@ struct Tile {
         }
     }
 
+    template <int S>
     __device__ __forceinline__ void store(char* stage) const {
         const int tid = threadIdx.x;
 #pragma unroll
@@ -193,13 +198,13 @@ struct Tile {
             if (A_UNITS % NT == 0 || u < A_UNITS) {
                 const int row = u / CPR, c = u % CPR;
                 if constexpr (MODE == 0) {
-                    *reinterpret_cast<f32x4*>(stage + chunk_offset(MODE, 0, BM, BK, row, c, 0)) = ra[i][0];
+                    *reinterpret_cast<f32x4*>(stage + chunk_offset(MODE, 0, BM, BK, row, c, 0)) = ra[S][i][0];
                 } else {
                     uint32_t w[TERMS][4];
 #pragma unroll
                     for (int q = 0; q < 4; ++q) {
                         uint32_t t[TERMS];
-                        split_pair<TERMS>(ra[i][q / 2][2 * (q % 2)], ra[i][q / 2][2 * (q % 2) + 1], t);
+                        split_pair<TERMS>(ra[S][i][q / 2][2 * (q % 2)], ra[S][i][q / 2][2 * (q % 2) + 1], t);
 #pragma unroll
                         for (int s = 0; s < TERMS; ++s) w[s][q] = t[s];
                     }
@@ -221,14 +226,14 @@ struct Tile {
                 for (int j = 0; j < BW; ++j) {
                     const int n = BW * nb + j;
                     if constexpr (MODE == 0) {
-                        const f32x4 v = {rb[i][0][j], rb[i][1][j], rb[i][2][j], rb[i][3][j]};
+                        const f32x4 v = {rb[S][i][0][j], rb[S][i][1][j], rb[S][i][2][j], rb[S][i][3][j]};
                         *reinterpret_cast<f32x4*>(bstage + chunk_offset(MODE, 1, BN, BK, n, c, 0)) = v;
                     } else {
                         uint32_t w[TERMS][4];
 #pragma unroll
                         for (int q = 0; q < 4; ++q) {
                             uint32_t t[TERMS];
-                            split_pair<TERMS>(rb[i][2 * q][j], rb[i][2 * q + 1][j], t);
+                            split_pair<TERMS>(rb[S][i][2 * q][j], rb[S][i][2 * q + 1][j], t);
 #pragma unroll
                             for (int s = 0; s < TERMS; ++s) w[s][q] = t[s];
                         }
@@ -345,29 +350,61 @@ struct Tile {
     // Main loop over the k-blocks of [kbeg, kend). ALL_FAST: every block is a
     // full block (vector loads, no bounds checks), so the steady-state
     // body is one basic block and the scheduler can interleave the MFMAs of
-    // stage kb with the split/store of stage kb+1. Registers for block kb+2 are
-    // loaded once the store of kb+1 has consumed them, i.e. one full iteration
-    // (plus the barrier) before they are needed.
+    // stage kb with the split/store of stage kb+1.
+    //
+    // One register set (PF = 1): registers for block kb+2 are loaded once the
+    // store of kb+1 has consumed them, one full iteration (plus the barrier)
+    // before they are needed. Two sets (PF = 2): the sets alternate and the
+    // loop is unrolled by two, so blocks kb+2 and kb+3 are both in flight
+    // while block kb is multiplied.
     template <bool ALL_FAST>
     __device__ __forceinline__ void mainloop(const float* __restrict__ A, const float* __restrict__ B,
                                              long long M, long long N, long long lda, long long ldb,
                                              long long m0, long long n0, long long kbeg, long long kend,
                                              bool full_mn, char* smem) {
         const long long nkb = (kend - kbeg + BK - 1) / BK;
-        auto fetch = [&](long long k0) {
-            load(A, B, M, N, lda, ldb, m0, n0, k0, kend, ALL_FAST || (full_mn && k0 + BK <= kend));
-        };
-        fetch(kbeg);
-        store(smem);
-        if (nkb > 1) fetch(kbeg + BK);
-        __syncthreads();
-        for (long long kb = 0; kb + 1 < nkb; ++kb) {
-            compute(smem + (kb & 1) * STAGE);
-            store(smem + ((kb + 1) & 1) * STAGE);
-            if (kb + 2 < nkb) fetch(kbeg + (kb + 2) * BK);
+        auto fast = [&](long long k0) { return ALL_FAST || (full_mn && k0 + BK <= kend); };
+        if constexpr (PF == 1) {
+            auto fetch = [&](long long k0) { load<0>(A, B, M, N, lda, ldb, m0, n0, k0, kend, fast(k0)); };
+            fetch(kbeg);
+            store<0>(smem);
+            if (nkb > 1) fetch(kbeg + BK);
             __syncthreads();
+            for (long long kb = 0; kb + 1 < nkb; ++kb) {
+                compute(smem + (kb & 1) * STAGE);
+                store<0>(smem + ((kb + 1) & 1) * STAGE);
+                if (kb + 2 < nkb) fetch(kbeg + (kb + 2) * BK);
+                __syncthreads();
+            }
+            compute(smem + ((nkb - 1) & 1) * STAGE);
+        } else {
+            auto fetch0 = [&](long long k0) { load<0>(A, B, M, N, lda, ldb, m0, n0, k0, kend, fast(k0)); };
+            auto fetch1 = [&](long long k0) { load<1>(A, B, M, N, lda, ldb, m0, n0, k0, kend, fast(k0)); };
+            // Loop invariant at even kb: stage 0 holds block kb, register set 1
+            // block kb+1, register set 0 block kb+2 (those that exist).
+            fetch0(kbeg);
+            if (nkb > 1) fetch1(kbeg + BK);
+            store<0>(smem);
+            if (nkb > 2) fetch0(kbeg + 2 * BK);
+            __syncthreads();
+            long long kb = 0;
+            for (; kb + 2 < nkb; kb += 2) {
+                compute(smem);
+                store<1>(smem + STAGE);
+                if (kb + 3 < nkb) fetch1(kbeg + (kb + 3) * BK);
+                __syncthreads();
+                compute(smem + STAGE);
+                store<0>(smem);
+                if (kb + 4 < nkb) fetch0(kbeg + (kb + 4) * BK);
+                __syncthreads();
+            }
+            compute(smem);
+            if (kb + 1 < nkb) {
+                store<1>(smem + STAGE);
+                __syncthreads();
+                compute(smem + STAGE);
+            }
         }
-        compute(smem + ((nkb - 1) & 1) * STAGE);
         __syncthreads();
     }
 

@@ -16,9 +16,12 @@
 
 namespace mmsk {
 
+// One register set of staging per geometry: the kernel's register allocation
+// is the maximum over its geometries, and two sets make the plain 128 x 256
+// case spill.
 template <int MODE, int CFG>
 using CfgTile = Tile<MODE, kTileCfgs[CFG].bm, kTileCfgs[CFG].bn, kTileCfgs[CFG].bk[MODE],
-                     kTileCfgs[CFG].wgm, kTileCfgs[CFG].wgn>;
+                     kTileCfgs[CFG].wgm, kTileCfgs[CFG].wgn, 1>;
 
 // The persistent kernel serves geometries 0..kGroupedMaxCfg (up to 128 x 256):
 // with the 256 x 256 geometry in the same kernel the register allocation of the

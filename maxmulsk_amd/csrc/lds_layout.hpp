@@ -75,6 +75,15 @@ MMSK_HD constexpr int b_unit_cols(int mode, int bn, int bk, int waves) {
     return waves == 8 ? 2 : (mode == 0 || per_thread >= 4 ? 4 : (per_thread >= 2 ? 2 : 1));
 }
 
+// Register sets of the tile kernel's global -> LDS staging, i.e. k-blocks in
+// flight (measured per mode on one MI355X): two for every plain tile (2-13%
+// faster from 512 x 512 x 4096 to 2048^3 and on the narrow shapes), two for
+// split6 tiles of up to 64 x 128, one for split3 (the extra registers cost it
+// occupancy and 1-6%).
+MMSK_HD constexpr int prefetch_sets(int mode, int bm, int bn) {
+    return mode == 0 ? 2 : (mode == 2 && bm * bn <= 64 * 128 ? 2 : 1);
+}
+
 // Bytes of one operand region (all terms) of one stage.
 MMSK_HD constexpr int operand_bytes(int mode, int rows, int bk) {
     return mode_terms(mode) * rows * bk * (mode == 0 ? 4 : 2);
