@@ -12,8 +12,11 @@
 #include <hip/hip_runtime.h>
 
 #include "gemm_tile.hpp"
+#include "plan.hpp"
 
 namespace mmsk {
+
+static_assert(kMaxSplit == 16, "sum_splits4 dispatches split counts 1..16");
 
 template <int S>
 __device__ __forceinline__ f32x4 sum_splits4_fixed(const float* __restrict__ P, long long stride) {
