@@ -54,9 +54,23 @@ static hipError_t launch_one(const GemmParams& p, hipStream_t stream) {
 }
 
 template <int MODE>
+static hipError_t launch_deep(const GemmParams& p, hipStream_t stream) {
+    constexpr TileCfg c = kTileCfgs[0];
+    const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
+    hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, kDeepBK[MODE], c.wgm, c.wgn>), dim3((unsigned)grid),
+                       dim3(64 * c.wgm * c.wgn), 0, stream, p);
+    return hipGetLastError();
+}
+
+template <int MODE>
 static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) {
     switch (cfg) {
-        case 0: return launch_one<MODE, 0>(p, stream);
+        case 0:
+            if constexpr (kDeepBK[MODE] > 0) {
+                const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
+                if (grid > 0 && grid <= kDeepMaxGrid) return launch_deep<MODE>(p, stream);
+            }
+            return launch_one<MODE, 0>(p, stream);
         case 1: return launch_one<MODE, 1>(p, stream);
         case 2: return launch_one<MODE, 2>(p, stream);
         case 3: return launch_one<MODE, 3>(p, stream);

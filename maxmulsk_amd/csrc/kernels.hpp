@@ -27,6 +27,18 @@ constexpr TileCfg kTileCfgs[kNumTileCfgs] = {
     {256, 256, 2, 4, {32, 32, 16}},
 };
 
+// Deep variant of geometry 0 (64 x 64): k-blocks of kDeepBK[mode] instead of
+// 32, launched when the whole grid fits in one workgroup per CU. Such launches
+// (small M and N, long K split into slices) run one k-block loop per CU, whose
+// time is a chain of per-block steps (load, split, store, barrier, MFMA);
+// deeper blocks halve or quarter the number of steps. With more workgroups
+// than CUs the larger LDS stages would cost occupancy instead. Slices keep the
+// boundaries of the 32-deep resolution, so results are bit-identical to the
+// regular kernel. 0 = no deep variant for that mode: split6 measured slower
+// with 64-deep blocks (its three BF16 terms make the stages large).
+constexpr int kDeepBK[3] = {64, 128, 0};
+constexpr int kDeepMaxGrid = 256;
+
 // One launch of the tiled kernel: an output grid of tiles_m x tiles_n tiles,
 // each computed over `splits` K slices of k_len elements. With splits > 1 every
 // slice writes its own FP32 partial matrix at C + split * c_split_stride and a

@@ -52,6 +52,31 @@ def test_every_split_count_reduces_exactly(mode):
             assert bad.size == 0, f"mode {mode} splits {s} tile {tile}: {len(bad)} wrong, first {bad[:4].tolist()}"
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("shape,plan", [((64, 8192, 512), (64, 16)), ((128, 32768, 512), (64, 16)),
+                                        ((100, 999, 300), (64, 4)), ((512, 4096, 512), (64, 4))],
+                         ids=["64x512x8192", "128x512x32768", "100x300x999", "512x512x4096"])
+def test_deep_small_grid_tile_matches_regular_tile(mode, shape, plan):
+    """A 64-tile launch of at most 256 workgroups runs the deep-k-block variant;
+    the grouped kernel always runs the regular 32-deep one. Same slices, same
+    summation order: the two must agree bit for bit on random data."""
+    torch = torch_cuda()
+    M, K, N = shape
+    tiles = -(-M // 64) * -(-N // 64)
+    assert tiles * plan[1] <= 256  # the deep variant is the one launched
+    A = orc.uniform(7000 + M, (M, K))
+    B = orc.uniform(8000 + N, (K, N))
+    single = gpu_gemm(A, B, mode=mode, plan=plan)
+    dA, dB, dC = Guarded(M * K, A), Guarded(K * N, B), Guarded(M * N, SENTINEL)
+    gp = mm.GroupedPlan([(dA.ptr, dB.ptr, dC.ptr, M, K, N)], mode=mode, plans=[plan])
+    gp.launch()
+    torch.cuda.synchronize()
+    gp.close()
+    grouped = dC.numpy((M, N))
+    assert np.array_equal(single, grouped), int((single != grouped).sum())
+    assert full_check(A, B, single)["bound_ratio"] <= 1.0
+
+
 # Plain FP32 MFMA on the measured shapes (M x N x K), checked against FP64 within
 # |C - C64| <= K * 2^-24 * |A||B| (gamma = 1).
 PLAIN_SHAPES = [
