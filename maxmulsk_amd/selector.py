@@ -117,7 +117,7 @@ class Trainer:
                                             self.v.data_ptr(), self.t, lr, 0.9, 0.999, 1e-8, None))
 
 
-def train(X: np.ndarray, T: np.ndarray, Mk: np.ndarray, steps: int = 4000, batch: int = 1024,
+def train(X: np.ndarray, T: np.ndarray, Mk: np.ndarray, steps: int = 20000, batch: int = 1024,
           lr: float = 3e-3, seed: int = 7, log=print) -> tuple[np.ndarray, list[float]]:
     import torch
     rng = np.random.default_rng(seed)
@@ -181,9 +181,11 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=5000)
     ap.add_argument("--eval", type=int, default=500)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--out", default=os.path.dirname(os.path.abspath(__file__)))
     ap.add_argument("--dataset", default="", help="reuse a collected dataset instead of timing")
+    ap.add_argument("--stage1", type=int, default=3, help="autotuner repetitions per candidate")
+    ap.add_argument("--stage2", type=int, default=6, help="autotuner repetitions per leader")
     args = ap.parse_args()
 
     def log(s):
@@ -195,7 +197,7 @@ def main() -> int:
     else:
         mnk, modes = draw_shapes(args.train + args.eval, seed=2027)
         t0 = time.time()
-        times = collect(mnk, modes, log=log)
+        times = collect(mnk, modes, stage1=args.stage1, stage2=args.stage2, log=log)
         log(f"[selector] collection took {time.time() - t0:.1f} s")
         np.savez_compressed(os.path.join(args.out, "selector_dataset.npz"), mnk=mnk, modes=modes, times=times,
                             n_train=args.train)
