@@ -33,9 +33,11 @@ constexpr TileCfg kTileCfgs[kNumTileCfgs] = {
 // time is a chain of per-block steps (load, split, store, barrier, MFMA);
 // deeper blocks halve or quarter the number of steps. With more workgroups
 // than CUs the larger LDS stages would cost occupancy instead. Slices keep the
-// boundaries of the 32-deep resolution, so results are bit-identical to the
-// regular kernel. 0 = no deep variant for that mode: split6 measured slower
-// with 64-deep blocks (its three BF16 terms make the stages large).
+// boundaries of the 32-deep resolution, so the products and their order are
+// those of the regular kernel; a slice shorter than one deep block only adds
+// MFMAs of zero padding, which can turn an exact -0 result into +0. 0 = no
+// deep variant for that mode: split6 measured slower with 64-deep blocks (its
+// three BF16 terms make the stages large).
 constexpr int kDeepBK[3] = {64, 128, 0};
 constexpr int kDeepMaxGrid = 256;
 
