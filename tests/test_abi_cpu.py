@@ -260,16 +260,7 @@ def _geometries():
     return [(g["cfg"], g["mode"]) for g in mm.tile_infos()]
 
 
-@pytest.mark.parametrize("cfg,mode", _geometries())
-def test_lds_access_patterns_are_bank_conflict_free(cfg, mode):
-    """Simulates the tile kernel's writer and reader lane patterns for every
-    geometry and mode: every group of 8 lanes of a store (128-byte LDS cycle)
-    and of 16 lanes of a fragment read (256-byte cycle) hits distinct 16-byte
-    bank groups. This is necessary, not sufficient: the B fragment reads pass
-    it, yet cost 4 conflict cycles per ds_read_b128 on the hardware, while the
-    A fragment reads and every store measured none."""
-    g = mm.tile_info(cfg, mode)
-    bm, bn, bk, nt, bw = g["bm"], g["bn"], g["bk"], g["threads"], g["b_cols"]
+def _check_tile_lds_patterns(mode, bm, bn, bk, nt, bw):
     lu = LU[mode]
     cpr = bk // lu
     insts = []
@@ -308,6 +299,29 @@ def test_lds_access_patterns_are_bank_conflict_free(cfg, mode):
         assert _worst_conflict(lanes, 8, 128) == 1
     for lanes in reads:
         assert _worst_conflict(lanes, 16, 256) == 1
+
+
+@pytest.mark.parametrize("cfg,mode", _geometries())
+def test_lds_access_patterns_are_bank_conflict_free(cfg, mode):
+    """Simulates the tile kernel's writer and reader lane patterns for every
+    geometry and mode: every group of 8 lanes of a store (128-byte LDS cycle)
+    and of 16 lanes of a fragment read (256-byte cycle) hits distinct 16-byte
+    bank groups. This is necessary, not sufficient: the B fragment reads pass
+    it, yet cost 4 conflict cycles per ds_read_b128 on the hardware, while the
+    A fragment reads and every store measured none."""
+    g = mm.tile_info(cfg, mode)
+    _check_tile_lds_patterns(mode, g["bm"], g["bn"], g["bk"], g["threads"], g["b_cols"])
+
+
+@pytest.mark.parametrize("mode,bm,bn,bk,threads,b_cols", [
+    (1, 128, 128, 32, 512, 2), (2, 128, 128, 32, 512, 2),  # eight-wave 128 x 128, split modes
+    (0, 64, 64, 64, 256, 4), (1, 64, 64, 128, 256, 4),     # deep 64 x 64 for small grids
+])
+def test_lds_patterns_of_the_launch_variants(mode, bm, bn, bk, threads, b_cols):
+    """Layouts the direct path launches besides the table's: the eight-wave
+    128 x 128 tile of the split modes (two-column B units, as every eight-wave
+    tile) and the deep 64 x 64 tiles (plain k-blocks of 64, split3 of 128)."""
+    _check_tile_lds_patterns(mode, bm, bn, bk, threads, b_cols)
 
 
 def test_tile_geometries():

@@ -77,6 +77,29 @@ def test_deep_small_grid_tile_matches_regular_tile(mode, shape, plan):
     assert full_check(A, B, single)["bound_ratio"] <= 1.0
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("shape,plan", [((300, 999, 1025), (128, 2)), ((256, 4096, 512), (128, 1)),
+                                        ((1000, 640, 700), (128, 4))],
+                         ids=["300x1025x999", "256x512x4096", "1000x700x640"])
+def test_eight_wave_128_tile_matches_grouped_tile(mode, shape, plan):
+    """Direct split-mode launches of the 128 x 128 tile run eight waves; the
+    grouped kernel runs the same geometry with four. The products and their
+    order are the same, so the results must agree bit for bit."""
+    torch = torch_cuda()
+    M, K, N = shape
+    A = orc.uniform(9000 + M, (M, K))
+    B = orc.uniform(9500 + N, (K, N))
+    single = gpu_gemm(A, B, mode=mode, plan=plan)
+    dA, dB, dC = Guarded(M * K, A), Guarded(K * N, B), Guarded(M * N, SENTINEL)
+    gp = mm.GroupedPlan([(dA.ptr, dB.ptr, dC.ptr, M, K, N)], mode=mode, plans=[plan])
+    gp.launch()
+    torch.cuda.synchronize()
+    gp.close()
+    grouped = dC.numpy((M, N))
+    assert np.array_equal(single, grouped), int((single != grouped).sum())
+    assert full_check(A, B, single)["bound_ratio"] <= 1.0
+
+
 # Plain FP32 MFMA on the measured shapes (M x N x K), checked against FP64 within
 # |C - C64| <= K * 2^-24 * |A||B| (gamma = 1).
 PLAIN_SHAPES = [
