@@ -184,8 +184,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--out", default=os.path.dirname(os.path.abspath(__file__)))
     ap.add_argument("--dataset", default="", help="reuse a collected dataset instead of timing")
-    ap.add_argument("--stage1", type=int, default=3, help="autotuner repetitions per candidate")
-    ap.add_argument("--stage2", type=int, default=6, help="autotuner repetitions per leader")
+    ap.add_argument("--stage1", type=int, default=5, help="autotuner repetitions per candidate")
+    ap.add_argument("--stage2", type=int, default=15, help="autotuner repetitions per leader")
     args = ap.parse_args()
 
     def log(s):
