@@ -130,7 +130,7 @@ maxmulsk_status build(const maxmulsk_gemm* problems, size_t n, int mode, const m
         if (!g.A || !g.B || !is_device_ptr(g.A, &da) || !is_device_ptr(g.B, &db) || da != dev || db != dev)
             return fail(MAXMULSK_ERR_UNSUPPORTED, "problem %zu: A/B null or not device memory on device %d", i, dev);
         const maxmulsk_plan pl = plans ? plans[i] : grouped_default(g.M, g.K, g.N, mode);
-        if (!resolve(g.M, g.K, g.N, mode, pl, &res[i], kGroupedMaxCfg))
+        if (!resolve(g.M, g.K, g.N, mode, pl, &res[i], kGroupedMaxCfg, grouped_min_cfg(mode)))
             return fail(MAXMULSK_ERR_UNSUPPORTED, "problem %zu: plan {%d,%d} rejected", i, pl.tile, pl.split_k);
         active[i] = true;
     }
@@ -394,7 +394,9 @@ extern "C" maxmulsk_status maxmulsk_grouped_autotune(const maxmulsk_gemm* proble
             const maxmulsk_plan cand = candidate(i);
             if (cand.tile == keep.tile && cand.split_k == keep.split_k) continue;
             Resolved r;
-            if (!resolve((size_t)std::get<0>(k), std::get<2>(k), std::get<1>(k), md, cand, &r, kGroupedMaxCfg)) continue;
+            if (!resolve((size_t)std::get<0>(k), std::get<2>(k), std::get<1>(k), md, cand, &r, kGroupedMaxCfg,
+                         grouped_min_cfg(md)))
+                continue;
             choice[k] = cand;
             float ms = 0;
             st = evaluate(&ms);

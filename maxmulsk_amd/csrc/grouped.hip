@@ -18,17 +18,18 @@ namespace mmsk {
 
 // One register set of staging per geometry: the kernel's register allocation
 // is the maximum over its geometries, and two sets make the plain 128 x 256
-// case spill.
+// case spill. Split modes: eight waves (2 x 4) per tile, 512 threads.
 template <int MODE, int CFG>
 using CfgTile = Tile<MODE, kTileCfgs[CFG].bm, kTileCfgs[CFG].bn, kTileCfgs[CFG].bk[MODE],
-                     kTileCfgs[CFG].wgm, kTileCfgs[CFG].wgn, 1>;
+                     MODE == 0 ? kTileCfgs[CFG].wgm : 2, MODE == 0 ? kTileCfgs[CFG].wgn : 4, 1>;
 
 // The persistent kernel serves geometries 0..kGroupedMaxCfg (up to 128 x 256):
 // with the 256 x 256 geometry in the same kernel the register allocation of the
 // combined kernel spills.
 template <int MODE>
 constexpr int max_smem() {
-    int m = CfgTile<MODE, 0>::SMEM;
+    int m = 0;
+    if constexpr (grouped_min_cfg(MODE) == 0) m = CfgTile<MODE, 0>::SMEM;
     m = m > CfgTile<MODE, 1>::SMEM ? m : CfgTile<MODE, 1>::SMEM;
     m = m > CfgTile<MODE, 2>::SMEM ? m : CfgTile<MODE, 2>::SMEM;
     m = m > CfgTile<MODE, 3>::SMEM ? m : CfgTile<MODE, 3>::SMEM;
@@ -53,7 +54,7 @@ __device__ __forceinline__ void run_item(const GroupProblem& p, int local, char*
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void grouped_kernel(GroupLaunch g) {
+__global__ __launch_bounds__(grouped_threads(MODE)) void grouped_kernel(GroupLaunch g) {
     __shared__ __attribute__((aligned(16))) char smem[max_smem<MODE>()];
     __shared__ int s_item;
     for (;;) {
@@ -65,7 +66,9 @@ __global__ __launch_bounds__(256) void grouped_kernel(GroupLaunch g) {
         const GroupProblem& p = g.problems[g.item_problem[item]];
         const int local = item - p.item_begin;
         switch (p.cfg) {
-            case 0: run_item<MODE, 0>(p, local, smem); break;
+            case 0:
+                if constexpr (grouped_min_cfg(MODE) == 0) run_item<MODE, 0>(p, local, smem);
+                break;
             case 1: run_item<MODE, 1>(p, local, smem); break;
             case 2: run_item<MODE, 2>(p, local, smem); break;
             case 3: run_item<MODE, 3>(p, local, smem); break;
@@ -93,9 +96,9 @@ __global__ __launch_bounds__(256) void grouped_reduce_kernel(const GroupProblem*
 hipError_t launch_grouped(int mode, const GroupLaunch& g, int grid, hipStream_t stream) {
     if (grid <= 0) return hipErrorInvalidValue;
     switch (mode) {
-        case 0: hipLaunchKernelGGL(grouped_kernel<0>, dim3(grid), dim3(256), 0, stream, g); break;
-        case 1: hipLaunchKernelGGL(grouped_kernel<1>, dim3(grid), dim3(256), 0, stream, g); break;
-        case 2: hipLaunchKernelGGL(grouped_kernel<2>, dim3(grid), dim3(256), 0, stream, g); break;
+        case 0: hipLaunchKernelGGL(grouped_kernel<0>, dim3(grid), dim3(grouped_threads(0)), 0, stream, g); break;
+        case 1: hipLaunchKernelGGL(grouped_kernel<1>, dim3(grid), dim3(grouped_threads(1)), 0, stream, g); break;
+        case 2: hipLaunchKernelGGL(grouped_kernel<2>, dim3(grid), dim3(grouped_threads(2)), 0, stream, g); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -105,9 +108,9 @@ int grouped_blocks_per_cu(int mode) {
     int n = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (mode) {
-        case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<0>, 256, 0); break;
-        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<1>, 256, 0); break;
-        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<2>, 256, 0); break;
+        case 0: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<0>, grouped_threads(0), 0); break;
+        case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<1>, grouped_threads(1), 0); break;
+        case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_kernel<2>, grouped_threads(2), 0); break;
         default: break;
     }
     if (e != hipSuccess) {

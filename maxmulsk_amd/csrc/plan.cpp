@@ -34,7 +34,7 @@ int cfg_for(int tile, size_t M) {
     }
 }
 
-bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* out, int max_cfg) {
+bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* out, int max_cfg, int min_cfg) {
     if (mode < 0 || mode > 2) return false;
     if (M == 0 || K == 0 || N == 0) return false;
     // Grid and index arithmetic inside the kernels is 32-bit for tile counts.
@@ -42,6 +42,7 @@ bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* 
     int cfg = cfg_for(p.tile, M);
     if (cfg < 0) return false;
     if (cfg > max_cfg) cfg = max_cfg;
+    if (cfg < min_cfg) cfg = min_cfg;
     if (p.split_k < 1 || p.split_k > kMaxSplit) return false;
     const TileCfg& tc = kTileCfgs[cfg];
     Resolved r{};

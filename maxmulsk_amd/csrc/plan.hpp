@@ -44,9 +44,10 @@ struct Resolved {
 // and the clamps of run_multiplication (:638-654): rejects a plan whose tile is
 // not 64/128/256 or whose split is outside 1..16, then clamps the split to the
 // shape (no empty slices). Shapes must be non-empty.
-// max_cfg caps the geometry (the grouped kernel serves geometries 0..4).
+// max_cfg caps the geometry (the grouped kernel serves geometries 0..4);
+// min_cfg raises it (the split-mode grouped kernel serves geometries 1..4).
 bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* out,
-             int max_cfg = 5);
+             int max_cfg = 5, int min_cfg = 0);
 
 // Shape rule (tier 3): the largest tile that still gives at least half a wave
 // of workgroups, then split-K until the grid covers the CUs, keeping slices at
