@@ -314,13 +314,15 @@ def test_lds_access_patterns_are_bank_conflict_free(cfg, mode):
 
 
 @pytest.mark.parametrize("mode,bm,bn,bk,threads,b_cols", [
-    (1, 128, 128, 32, 512, 2), (2, 128, 128, 32, 512, 2),  # eight-wave 128 x 128, split modes
+    # eight-wave tiles of the split modes: direct 128 x 128, grouped 64 x 128 .. 128 x 256
+    *[(m, bm, bn, 32, 512, 2) for m in (1, 2) for bm, bn in ((128, 128), (64, 128), (64, 256), (128, 256))],
     (0, 64, 64, 64, 256, 4), (1, 64, 64, 128, 256, 4),     # deep 64 x 64 for small grids
 ])
 def test_lds_patterns_of_the_launch_variants(mode, bm, bn, bk, threads, b_cols):
-    """Layouts the direct path launches besides the table's: the eight-wave
-    128 x 128 tile of the split modes (two-column B units, as every eight-wave
-    tile) and the deep 64 x 64 tiles (plain k-blocks of 64, split3 of 128)."""
+    """Layouts launched besides the table's: the eight-wave tiles of the split
+    modes (two-column B units, as every eight-wave tile; the direct 128 x 128
+    tile and every tile of the grouped kernel) and the deep 64 x 64 tiles
+    (plain k-blocks of 64, split3 of 128)."""
     _check_tile_lds_patterns(mode, bm, bn, bk, threads, b_cols)
 
 
