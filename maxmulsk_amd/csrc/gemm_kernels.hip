@@ -76,7 +76,7 @@ static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) 
         case 3:
             if constexpr (MODE != 0) {
                 constexpr TileCfg c = kTileCfgs[3];
-                constexpr int WGM = kSplitCfg3Waves[0], WGN = kSplitCfg3Waves[1];
+                constexpr int WGM = kSplitWideWaves[0], WGN = kSplitWideWaves[1];
                 const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
                 if (grid <= 0 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
                 hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, c.bk[MODE], WGM, WGN>), dim3((unsigned)grid),
@@ -84,7 +84,17 @@ static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) 
                 return hipGetLastError();
             }
             return launch_one<MODE, 3>(p, stream);
-        case 4: return launch_one<MODE, 4>(p, stream);
+        case 4:
+            if constexpr (MODE != 0) {
+                constexpr TileCfg c = kTileCfgs[4];
+                constexpr int WGM = kSplitWideWaves[0], WGN = kSplitWideWaves[1];
+                const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
+                if (grid <= 0 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
+                hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, c.bk[MODE], WGM, WGN>), dim3((unsigned)grid),
+                                   dim3(64 * WGM * WGN), 0, stream, p);
+                return hipGetLastError();
+            }
+            return launch_one<MODE, 4>(p, stream);
         default: return hipErrorInvalidValue;  // geometry 5 runs on the packed path
     }
 }

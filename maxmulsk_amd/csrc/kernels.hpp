@@ -41,13 +41,14 @@ constexpr TileCfg kTileCfgs[kNumTileCfgs] = {
 constexpr int kDeepBK[3] = {64, 128, 0};
 constexpr int kDeepMaxGrid = 256;
 
-// Direct launches of geometry 3 (128 x 128) in the split modes run eight waves
-// (2 x 4, 64 x 32 each) instead of the table's four: twice the waves per SIMD
-// to cover LDS and barrier latency, measured 2-7% faster (2048^3 split3 71.0 ->
-// 67.0 us). Plain mode keeps four (eight measured 4-9% slower), and so does
-// the grouped kernel, whose workgroups are 256 threads. The wave layout changes
-// no product and no summation order, so results are identical.
-constexpr int kSplitCfg3Waves[2] = {2, 4};
+// Direct launches of geometries 3 (128 x 128) and 4 (128 x 256) in the split
+// modes run eight waves (2 x 4) instead of the table's four: twice the waves
+// per SIMD to cover LDS and barrier latency. Measured 2-7% faster on 128 x 128
+// (2048^3 split3 71.0 -> 67.0 us) and up to 14% on 128 x 256 (96 x 8192 x 8192
+// split6 117.6 -> 100.6 us). Plain mode keeps four (eight measured 4-9%
+// slower). The wave layout changes no product and no summation order, so
+// results are identical to the four-wave tiles.
+constexpr int kSplitWideWaves[2] = {2, 4};
 
 // One launch of the tiled kernel: an output grid of tiles_m x tiles_n tiles,
 // each computed over `splits` K slices of k_len elements. With splits > 1 every
