@@ -186,6 +186,9 @@ def main() -> int:
     ap.add_argument("--dataset", default="", help="reuse a collected dataset instead of timing")
     ap.add_argument("--stage1", type=int, default=5, help="autotuner repetitions per candidate")
     ap.add_argument("--stage2", type=int, default=15, help="autotuner repetitions per leader")
+    ap.add_argument("--seeds", default="7,11,13,17",
+                    help="initialisation seeds; with several, the one with the lowest regret on a "
+                         "validation split of the training shapes (the last 500) is trained on all of them")
     args = ap.parse_args()
 
     def log(s):
@@ -202,16 +205,31 @@ def main() -> int:
         np.savez_compressed(os.path.join(args.out, "selector_dataset.npz"), mnk=mnk, modes=modes, times=times,
                             n_train=args.train)
     n_train = args.train
+    seeds = [int(x) for x in args.seeds.split(",") if x]
+    seed = seeds[0]
+    val_regret = {}
+    if len(seeds) > 1:
+        # seed selection on a validation split of the training shapes; the
+        # held-out shapes take no part in it
+        n_fit = n_train - 500
+        Xf = features(mnk[:n_fit], modes[:n_fit])
+        Tf, Mf = targets(times[:n_fit])
+        for sd in seeds:
+            p_s, _ = train(Xf, Tf, Mf, steps=args.steps, seed=sd, log=lambda _s: None)
+            val_regret[sd] = evaluate(p_s, mnk[n_fit:n_train], modes[n_fit:n_train], times[n_fit:n_train])["mlp"]["mean"]
+            log(f"[selector] seed {sd}: validation regret {val_regret[sd]:.4f}")
+        seed = min(val_regret, key=val_regret.get)
     X = features(mnk[:n_train], modes[:n_train])
     T, Mk = targets(times[:n_train])
     t0 = time.time()
-    params, hist = train(X, T, Mk, steps=args.steps, log=log)
+    params, hist = train(X, T, Mk, steps=args.steps, seed=seed, log=log)
     train_s = time.time() - t0
     ev = evaluate(params, mnk[n_train:], modes[n_train:], times[n_train:])
     tr = evaluate(params, mnk[:n_train], modes[:n_train], times[:n_train])
     path = os.path.join(args.out, "maxmulsk_selector.bin")
     mm.check(mm.lib().maxmulsk_selector_save(path.encode(), params.ctypes.data))
     report = {"train_shapes": n_train, "eval_shapes": int(len(mnk) - n_train), "steps": args.steps,
+              "seed": seed, "validation_regret_by_seed": val_regret,
               "train_seconds": train_s, "loss_history": hist,
               "heldout": {k: v for k, v in ev.items() if k != "mlp_choice"},
               "train": {k: v for k, v in tr.items() if k != "mlp_choice"}}
