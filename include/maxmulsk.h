@@ -278,8 +278,9 @@ const char* maxmulsk_last_error(void);
  * of range. */
 int maxmulsk_lds_chunk_offset(int mode, int op, int rows, int bk, int row, int c, int term);
 /* Staging geometry of tile geometry cfg (0..maxmulsk_num_tile_geometries()-1)
- * in a mode: out[0..5] = BM, BN, BK, threads per workgroup, columns per B
- * staging unit, waves. Returns 0, or -1 for an unknown cfg or mode. */
+ * in a mode, as one-call launches run it: out[0..5] = BM, BN, BK, threads per
+ * workgroup, columns per B staging unit, waves (eight for the 128-row tiles of
+ * the split modes). Returns 0, or -1 for an unknown cfg or mode. */
 int maxmulsk_num_tile_geometries(void);
 int maxmulsk_tile_info(int cfg, int mode, int* out);
 

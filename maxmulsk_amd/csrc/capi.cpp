@@ -162,7 +162,7 @@ extern "C" int maxmulsk_num_tile_geometries(void) { return kNumTileCfgs; }
 extern "C" int maxmulsk_tile_info(int cfg, int mode, int* out) {
     if (cfg < 0 || cfg >= kNumTileCfgs || mode < 0 || mode > 2 || !out) return -1;
     const TileCfg& t = kTileCfgs[cfg];
-    const int waves = t.wgm * t.wgn;
+    const int waves = cfg < 5 ? direct_wgm(mode, cfg) * direct_wgn(mode, cfg) : t.wgm * t.wgn;
     out[0] = t.bm;
     out[1] = t.bn;
     out[2] = t.bk[mode];

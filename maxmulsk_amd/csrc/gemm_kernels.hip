@@ -46,10 +46,11 @@ template <int MODE, int CFG>
 static hipError_t launch_one(const GemmParams& p, hipStream_t stream) {
     constexpr TileCfg c = kTileCfgs[CFG];
     constexpr int BK = c.bk[MODE];
+    constexpr int WGM = direct_wgm(MODE, CFG), WGN = direct_wgn(MODE, CFG);
     const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
     if (grid <= 0 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, BK, c.wgm, c.wgn>), dim3((unsigned)grid),
-                       dim3(64 * c.wgm * c.wgn), 0, stream, p);
+    hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, BK, WGM, WGN>), dim3((unsigned)grid),
+                       dim3(64 * WGM * WGN), 0, stream, p);
     return hipGetLastError();
 }
 
@@ -73,28 +74,8 @@ static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) 
             return launch_one<MODE, 0>(p, stream);
         case 1: return launch_one<MODE, 1>(p, stream);
         case 2: return launch_one<MODE, 2>(p, stream);
-        case 3:
-            if constexpr (MODE != 0) {
-                constexpr TileCfg c = kTileCfgs[3];
-                constexpr int WGM = kSplitWideWaves[0], WGN = kSplitWideWaves[1];
-                const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
-                if (grid <= 0 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
-                hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, c.bk[MODE], WGM, WGN>), dim3((unsigned)grid),
-                                   dim3(64 * WGM * WGN), 0, stream, p);
-                return hipGetLastError();
-            }
-            return launch_one<MODE, 3>(p, stream);
-        case 4:
-            if constexpr (MODE != 0) {
-                constexpr TileCfg c = kTileCfgs[4];
-                constexpr int WGM = kSplitWideWaves[0], WGN = kSplitWideWaves[1];
-                const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
-                if (grid <= 0 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
-                hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, c.bk[MODE], WGM, WGN>), dim3((unsigned)grid),
-                                   dim3(64 * WGM * WGN), 0, stream, p);
-                return hipGetLastError();
-            }
-            return launch_one<MODE, 4>(p, stream);
+        case 3: return launch_one<MODE, 3>(p, stream);
+        case 4: return launch_one<MODE, 4>(p, stream);
         default: return hipErrorInvalidValue;  // geometry 5 runs on the packed path
     }
 }

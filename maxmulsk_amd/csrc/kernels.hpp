@@ -50,6 +50,12 @@ constexpr int kDeepMaxGrid = 256;
 // results are identical to the four-wave tiles.
 constexpr int kSplitWideWaves[2] = {2, 4};
 
+// Wave layout (rows x columns of waves) of the direct launch of geometry cfg
+// (0..4) in a mode.
+constexpr bool direct_wide(int mode, int cfg) { return mode != 0 && (cfg == 3 || cfg == 4); }
+constexpr int direct_wgm(int mode, int cfg) { return direct_wide(mode, cfg) ? kSplitWideWaves[0] : kTileCfgs[cfg].wgm; }
+constexpr int direct_wgn(int mode, int cfg) { return direct_wide(mode, cfg) ? kSplitWideWaves[1] : kTileCfgs[cfg].wgn; }
+
 // One launch of the tiled kernel: an output grid of tiles_m x tiles_n tiles,
 // each computed over `splits` K slices of k_len elements. With splits > 1 every
 // slice writes its own FP32 partial matrix at C + split * c_split_stride and a

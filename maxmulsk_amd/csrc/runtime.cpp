@@ -227,13 +227,16 @@ maxmulsk_status run_gemm(const GemmCall& c) {
     if (c.M > 0x7fffffffULL || c.N > 0x7fffffffULL || c.K > 0x7fffffffULL ||
         c.M * c.N > (size_t)1 << 40)
         return fail(MAXMULSK_ERR_UNSUPPORTED, "shape too large");
+    // C is classified even without a usable device: device memory of another
+    // GPU must not reach the host memset below.
     int cdev = -1;
-    const bool c_dev = usable_devices() > 0 && is_device_ptr(c.C, &cdev);
+    const bool c_dev = is_device_ptr(c.C, &cdev);
     if (c.K == 0) {
         if (!c_dev) {
             std::memset(c.C, 0, c.M * c.N * sizeof(float));
             return MAXMULSK_OK;
         }
+        if (!device_usable(cdev)) return fail(MAXMULSK_ERR_UNSUPPORTED, "device %d is not gfx950", cdev);
         DeviceGuard g(cdev);
         count(kCntZeroFill);
         if (launch_fill_zero_f32(c.C, c.M * c.N, c.stream) != hipSuccess ||
