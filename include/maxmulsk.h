@@ -154,7 +154,10 @@ maxmulsk_status maxmulsk_sgemm_grouped(const maxmulsk_gemm* problems, size_t n,
  * grouped autotuner cache, else the grouped rule; see
  * maxmulsk_grouped_default_plan), or hold one plan
  * per problem. In a grouped launch a 256-wide tile on more than 128 rows runs
- * on the 128 x 256 geometry. */
+ * on the 128 x 256 geometry, and in the split modes a 64-wide tile runs on the
+ * 64 x 128 geometry (same K slices). A plan owns its work counter and split-K
+ * workspace: launches of one plan must not overlap on the device (launch it on
+ * one stream, or order the streams); different plans are independent. */
 typedef struct maxmulsk_grouped_plan_s* maxmulsk_grouped_plan;
 maxmulsk_status maxmulsk_grouped_create(const maxmulsk_gemm* problems, size_t n,
                                         maxmulsk_mode mode, const maxmulsk_plan* plans,
