@@ -33,8 +33,12 @@ def as_args(probs):
     return [(a.ptr, b.ptr, c.ptr, M, K, N) for a, b, c, M, K, N in probs]
 
 
-def cfg_capped(M, plan):
-    return plan[0] == 256 and M > 128
+def standalone_plan(M, plan):
+    """Plan of a standalone call that computes every element as a grouped
+    launch of `plan` does: a 256-wide tile on more than 128 rows runs on
+    128-row tiles there, so the standalone call uses 128-wide tiles (the same
+    K slices)."""
+    return (128, plan[1]) if plan[0] == 256 and M > 128 else plan
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
@@ -54,15 +58,40 @@ def test_grouped_matches_oracle_and_standalone(mode):
             assert np.all(Cg == 0.0)
             continue
         assert full_check(A, B, Cg)["bound_ratio"] <= 1.0, (M, K, N)
-        plan = mm.grouped_default_plan(M, K, N, mode)
-        if not cfg_capped(M, plan):
-            # same tile code, same slices, same reduce order (the split-mode
-            # grouped kernels run 64-tile plans on 64 x 128 tiles with eight
-            # waves: other wave tiles, same products)
-            ref = Guarded(M * N, SENTINEL)
-            assert mm.sgemm_ex(dA.ptr, dB.ptr, ref.ptr, M, K, N, mode=mode, plan=plan) == mm.OK
-            torch.cuda.synchronize()
-            assert np.array_equal(ref.numpy((M, N)), Cg), (M, K, N, plan)
+        plan = standalone_plan(M, mm.grouped_default_plan(M, K, N, mode))
+        # same slices, same products per element, same reduce order (the
+        # split-mode grouped kernels run 64-tile plans on 64 x 128 tiles with
+        # eight waves: other wave tiles, same products)
+        ref = Guarded(M * N, SENTINEL)
+        assert mm.sgemm_ex(dA.ptr, dB.ptr, ref.ptr, M, K, N, mode=mode, plan=plan) == mm.OK
+        torch.cuda.synchronize()
+        assert np.array_equal(ref.numpy((M, N)), Cg), (M, K, N, plan)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_grouped_tail_rows_match_standalone(mode):
+    """In plain mode, problems on 128-row tiles whose last tile holds at most
+    64 rows run as two row ranges (128-row tiles, then one 64-row tile); the
+    split modes keep the padded tile. Either way every row gets the standalone
+    call's K slices and products: bit-identical results, the bound, and
+    nothing written outside C."""
+    torch = torch_cuda()
+    specs = [(129, 700, 256), (192, 1536, 130), (193, 300, 200), (255, 513, 260), (320, 2048, 512),
+             (130, 4100, 384)]
+    plans = [(256, 2), (128, 1), (128, 4), (256, 1), (256, 4), (128, 16)]
+    probs, host = build(specs, 900)
+    gp = mm.GroupedPlan(as_args(probs), mode=mode, plans=plans)
+    gp.launch()
+    torch.cuda.synchronize()
+    gp.close()
+    for (dA, dB, dC, M, K, N), (A, B), plan in zip(probs, host, plans):
+        assert dC.guards_intact(), (M, K, N)
+        Cg = dC.numpy((M, N))
+        assert full_check(A, B, Cg)["bound_ratio"] <= 1.0, (M, K, N)
+        ref = Guarded(M * N, SENTINEL)
+        assert mm.sgemm_ex(dA.ptr, dB.ptr, ref.ptr, M, K, N, mode=mode, plan=standalone_plan(M, plan)) == mm.OK
+        torch.cuda.synchronize()
+        assert np.array_equal(ref.numpy((M, N)), Cg), (mode, M, K, N, plan)
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
