@@ -143,7 +143,8 @@ typedef struct {
  * is validated before anything runs; the first failure is returned and no C is
  * touched (the validate-all-first contract of run_batch,
  * sme/v4/sme-1x4-v4c-onestream.cpp:683-694). opt->plan is ignored: each problem
- * takes its own plan from maxmulsk_choose_plan. */
+ * takes the grouped default plan (see maxmulsk_grouped_default_plan). Returns
+ * with every C complete; opt->async is ignored. */
 maxmulsk_status maxmulsk_sgemm_grouped(const maxmulsk_gemm* problems, size_t n,
                                        const maxmulsk_options* opt);
 
@@ -171,7 +172,10 @@ void maxmulsk_grouped_destroy(maxmulsk_grouped_plan plan);
 
 /* Plan a problem gets inside a grouped launch when none is given: the grouped
  * autotuner's choice for its shape class if there is one, else the grouped
- * rule (tile edge from N, K sliced into pieces of at most 4096). */
+ * rule (tile edge from N, K sliced into pieces of at most 4096). This is the
+ * plan of a problem alone in its launch: in a launch of at least 1024 tiles
+ * (one K slice each) the rule's problems run unsliced, unless one of their
+ * tiles holds more work than the launch's total divided by 256 CUs. */
 maxmulsk_plan maxmulsk_grouped_default_plan(size_t M, size_t K, size_t N, maxmulsk_mode mode);
 
 /* Grouped autotuner: shape class = (M rounded up to 64/128/256/512+, N, K).
