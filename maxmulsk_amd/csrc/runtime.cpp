@@ -1,10 +1,15 @@
 #include "runtime.hpp"
 
+#include <algorithm>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <vector>
+
+#include <unistd.h>
 
 #include "kernels.hpp"
 #include "trace.hpp"
@@ -90,6 +95,16 @@ bool is_device_ptr(const void* p, int* dev) {
     return false;
 }
 
+bool is_pinned_host(const void* p) {
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof attr);
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeHost;
+}
+
 DeviceGuard::DeviceGuard(int dev) {
     if (hipGetDevice(&prev) != hipSuccess) {
         (void)hipGetLastError();
@@ -125,6 +140,173 @@ bool grow(void** buf, size_t* cap, size_t bytes) {
     *cap = want;
     return true;
 }
+
+namespace {
+
+// Host threads that split large copies between pageable caller memory and the
+// pinned bounce buffers. One thread copies about 31 GB/s on an MI355X host,
+// below the 53 GB/s of a pinned DMA; four reach about 64 GB/s. The calling
+// thread copies one part itself. Created on first use and never destroyed
+// (they wait on a condition variable for the life of the process). A child
+// process of a fork has no workers and copies alone.
+class CopyWorkers {
+  public:
+    explicit CopyWorkers(int workers) : pid_(getpid()) {
+        int started = 0;
+        try {
+            for (int w = 0; w < workers; ++w, ++started) std::thread([this, w] { loop(w + 1); }).detach();
+        } catch (...) {
+        }
+        parts_ = started + 1;
+    }
+
+    void copy(void* dst, const void* src, size_t n) {
+        if (n < kParallelMin || parts_ == 1 || getpid() != pid_) {
+            std::memcpy(dst, src, n);
+            return;
+        }
+        std::lock_guard<std::mutex> call(call_mu_);  // one copy at a time
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            dst_ = static_cast<char*>(dst);
+            src_ = static_cast<const char*>(src);
+            n_ = n;
+            pending_ = parts_ - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        part(0);
+        std::unique_lock<std::mutex> l(mu_);
+        done_.wait(l, [&] { return pending_ == 0; });
+    }
+
+  private:
+    static constexpr size_t kParallelMin = 1u << 20;
+
+    // Slice i of the current copy (4 KiB-aligned boundaries).
+    void part(int i) const {
+        const size_t per = (n_ / parts_ + 4095) & ~size_t(4095);
+        const size_t b = std::min(n_, per * (size_t)i), e = std::min(n_, b + per);
+        if (e > b) std::memcpy(dst_ + b, src_ + b, e - b);
+    }
+
+    void loop(int idx) {
+        unsigned long long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> l(mu_);
+                cv_.wait(l, [&] { return gen_ != seen; });
+                seen = gen_;
+            }
+            part(idx);
+            std::lock_guard<std::mutex> g(mu_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+
+    const pid_t pid_;
+    int parts_ = 1;
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_;
+    unsigned long long gen_ = 0;
+    int pending_ = 0;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t n_ = 0;
+};
+
+CopyWorkers& copy_workers() {
+    static CopyWorkers* w = [] {
+        const unsigned hc = std::thread::hardware_concurrency();
+        return new CopyWorkers(hc >= 8 ? 3 : (hc >= 2 ? (int)hc / 2 - 1 : 0));
+    }();
+    return *w;
+}
+
+constexpr size_t kBounceBytes = 8u << 20;  // per bounce buffer
+
+// Pinned bounce buffers of the device (caller holds pl.mu). False when pinned
+// memory is unavailable; callers then copy pageable memory directly.
+bool ensure_bounce(DevicePools& pl) {
+    if (pl.bounce_failed) return false;
+    if (pl.bounce[0]) return true;
+    bool ok = true;
+    for (int b = 0; b < 2 && ok; ++b) {
+        ok = hipHostMalloc(&pl.bounce[b], kBounceBytes, hipHostMallocDefault) == hipSuccess &&
+             hipEventCreateWithFlags(&pl.bounce_ev[b], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {
+        (void)hipGetLastError();
+        for (int b = 0; b < 2; ++b) {
+            if (pl.bounce[b]) (void)hipHostFree(pl.bounce[b]);
+            if (pl.bounce_ev[b]) (void)hipEventDestroy(pl.bounce_ev[b]);
+            pl.bounce[b] = nullptr;
+            pl.bounce_ev[b] = nullptr;
+        }
+        pl.bounce_failed = true;
+    }
+    return ok;
+}
+
+// Host -> device copy of a caller operand on `stream`. Pageable memory goes
+// through the two bounce buffers: the host fills one while the DMA of the
+// other runs. Returns when the host has consumed `src` (DMAs may still run).
+hipError_t stage_in(DevicePools& pl, void* dst, const void* src, size_t n, hipStream_t stream) {
+    if (is_pinned_host(src) || !ensure_bounce(pl))
+        return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, stream);
+    CopyWorkers& cw = copy_workers();
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+    for (size_t off = 0, i = 0; off < n; off += kBounceBytes, ++i) {
+        const int b = (int)(i & 1);
+        const size_t len = std::min(kBounceBytes, n - off);
+        if (pl.bounce_used[b]) {
+            const hipError_t e = hipEventSynchronize(pl.bounce_ev[b]);
+            if (e != hipSuccess) return e;
+        }
+        cw.copy(pl.bounce[b], s + off, len);
+        hipError_t e = hipMemcpyAsync(d + off, pl.bounce[b], len, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipEventRecord(pl.bounce_ev[b], stream);
+        if (e != hipSuccess) return e;
+        pl.bounce_used[b] = true;
+    }
+    return hipSuccess;
+}
+
+// Device -> host copy of C on `stream` (after the work queued before it).
+// Pageable memory: the DMA fills one bounce buffer while the host empties the
+// other. Returns with `dst` complete.
+hipError_t stage_out(DevicePools& pl, void* dst, const void* src, size_t n, hipStream_t stream) {
+    if (is_pinned_host(dst) || !ensure_bounce(pl)) {
+        const hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, stream);
+        return e == hipSuccess ? hipStreamSynchronize(stream) : e;
+    }
+    CopyWorkers& cw = copy_workers();
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+    const size_t chunks = (n + kBounceBytes - 1) / kBounceBytes;
+    auto issue = [&](size_t i) {
+        const int b = (int)(i & 1);
+        const size_t off = i * kBounceBytes, len = std::min(kBounceBytes, n - off);
+        hipError_t e = hipMemcpyAsync(pl.bounce[b], s + off, len, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipEventRecord(pl.bounce_ev[b], stream);
+        pl.bounce_used[b] = true;
+        return e;
+    };
+    hipError_t e = issue(0);
+    if (e == hipSuccess && chunks > 1) e = issue(1);
+    for (size_t i = 0; e == hipSuccess && i < chunks; ++i) {
+        const int b = (int)(i & 1);
+        const size_t off = i * kBounceBytes, len = std::min(kBounceBytes, n - off);
+        e = hipEventSynchronize(pl.bounce_ev[b]);
+        if (e != hipSuccess) break;
+        cw.copy(d + off, pl.bounce[b], len);
+        if (i + 2 < chunks) e = issue(i + 2);
+    }
+    return e;
+}
+
+}  // namespace
 
 maxmulsk_status pool_workspace_acquire(DevicePools& pl, size_t bytes, hipStream_t stream, void** ws) {
     if (!grow(&pl.ws, &pl.ws_cap, bytes)) return fail(MAXMULSK_ERR_ALLOC, "workspace allocation of %zu bytes failed", bytes);
@@ -309,7 +491,7 @@ maxmulsk_status run_gemm(const GemmCall& c) {
         if (!grow(&pl.stage, &pl.stage_cap, need)) return fail(MAXMULSK_ERR_ALLOC, "staging allocation of %zu bytes failed", need);
         char* s = static_cast<char*>(pl.stage);
         if (!a_dev) {
-            if (hipMemcpyAsync(s, c.A, a_bytes, hipMemcpyHostToDevice, c.stream) != hipSuccess) {
+            if (stage_in(pl, s, c.A, a_bytes, c.stream) != hipSuccess) {
                 (void)hipGetLastError();
                 return fail(MAXMULSK_ERR_DEVICE, "H2D copy of A failed");
             }
@@ -318,7 +500,7 @@ maxmulsk_status run_gemm(const GemmCall& c) {
             s += up(a_bytes);
         }
         if (!b_dev) {
-            if (hipMemcpyAsync(s, c.B, b_bytes, hipMemcpyHostToDevice, c.stream) != hipSuccess) {
+            if (stage_in(pl, s, c.B, b_bytes, c.stream) != hipSuccess) {
                 (void)hipGetLastError();
                 return fail(MAXMULSK_ERR_DEVICE, "H2D copy of B failed");
             }
@@ -344,7 +526,7 @@ maxmulsk_status run_gemm(const GemmCall& c) {
     if (e == hipSuccess && pooled) e = pool_workspace_release(pl, c.stream);
     if (e == hipSuccess && staged && !c_dev) {
         Scope scope("maxmulsk/writeback");
-        e = hipMemcpyAsync(c.C, C, c.M * c.N * sizeof(float), hipMemcpyDeviceToHost, c.stream);
+        e = stage_out(pl, c.C, C, c.M * c.N * sizeof(float), c.stream);
         count(kCntWriteback);
     }
     if (e == hipSuccess && !c.async) {

@@ -25,6 +25,8 @@ std::string device_name(int dev);
 
 // True when p is device (or managed) memory; *dev receives its device.
 bool is_device_ptr(const void* p, int* dev);
+// True when p is pinned host memory (hipHostMalloc), which DMA reads directly.
+bool is_pinned_host(const void* p);
 
 // Makes `dev` current for the lifetime of the guard.
 struct DeviceGuard {
@@ -46,6 +48,12 @@ struct DevicePools {
     bool ws_event_recorded = false;
     void* stage = nullptr;
     size_t stage_cap = 0;
+    // Pinned bounce buffers between pageable caller memory and the staging
+    // buffer, with the event of each one's last copy (synchronous calls only).
+    void* bounce[2] = {nullptr, nullptr};
+    hipEvent_t bounce_ev[2] = {nullptr, nullptr};
+    bool bounce_used[2] = {false, false};
+    bool bounce_failed = false;  // pinned allocation failed once: copy pageable memory directly
     // Stream and events of the autotuner, created on first use and kept (a
     // stream holds a hardware queue; creating one per tuning call is costly).
     hipStream_t tune_stream = nullptr;

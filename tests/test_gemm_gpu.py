@@ -200,6 +200,43 @@ def test_v3_v4_and_host_pointers_agree():
         assert full_check(A, B, Cx)["bound_ratio"] <= 1.0
 
 
+def test_host_operands_larger_than_the_staging_chunks():
+    """Host operands that span several 8 MiB staging chunks, with ragged
+    sizes, in pageable and pinned host memory, mixed with device operands:
+    every combination gives the device-pointer result bit for bit and leaves
+    the device operands and C's guards intact."""
+    torch = torch_cuda()
+    M, N, K = 1500, 1900, 1700  # A 10.2 MB, B 12.9 MB, C 11.4 MB
+    A = orc.uniform(71, (M, K))
+    B = orc.uniform(72, (K, N))
+    plan = (128, 1)
+    dA, dB = Guarded(M * K, A), Guarded(K * N, B)
+    dC = Guarded(M * N, SENTINEL)
+    assert mm.sgemm_ex(dA.ptr, dB.ptr, dC.ptr, M, K, N, mode=1, plan=plan) == mm.OK, mm.last_error()
+    torch.cuda.synchronize()
+    ref = dC.numpy((M, N)).copy()
+    assert full_check(A, B, ref)["bound_ratio"] <= 1.0
+    pA = torch.from_numpy(A).pin_memory()
+    pB = torch.from_numpy(B).pin_memory()
+    pC = torch.empty((M, N), dtype=torch.float32).pin_memory()
+    hC = np.empty((M, N), np.float32)
+    cases = [("pageable A, B, C", A, B, hC), ("pinned A, pageable B, pinned C", pA, B, pC),
+             ("pageable A, device B, pageable C", A, dB.ptr, hC), ("device A, pinned B, device C", dA.ptr, pB, dC)]
+    for name, a, b, c in cases:
+        if c is dC:
+            dC.view.fill_(SENTINEL)
+            torch.cuda.synchronize()
+        elif isinstance(c, np.ndarray):
+            c.fill(SENTINEL)
+        else:
+            c.fill_(SENTINEL)
+        assert mm.sgemm_ex(a, b, c.ptr if c is dC else c, M, K, N, mode=1, plan=plan) == mm.OK, (name, mm.last_error())
+        got = dC.numpy((M, N)) if c is dC else (c if isinstance(c, np.ndarray) else c.numpy())
+        assert np.array_equal(got, ref), name
+    assert dA.guards_intact() and dB.guards_intact() and dC.guards_intact()
+    assert np.array_equal(dA.numpy((M, K)), A) and np.array_equal(dB.numpy((K, N)), B)
+
+
 def test_repeat_call_with_new_B_same_buffers():
     """check_nb.cpp:61-66: nothing stale may survive between calls."""
     torch = torch_cuda()
