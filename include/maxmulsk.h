@@ -177,6 +177,11 @@ void maxmulsk_grouped_destroy(maxmulsk_grouped_plan plan);
  * (one K slice each) the rule's problems run unsliced, unless one of their
  * tiles holds more work than the launch's total divided by 256 CUs. */
 maxmulsk_plan maxmulsk_grouped_default_plan(size_t M, size_t K, size_t N, maxmulsk_mode mode);
+/* The plans a grouped launch of these problems takes when none are given, one
+ * per problem (out: n entries; {0, 0} for problems with M, N or K = 0). Pure:
+ * reads only the sizes, runs nothing. */
+maxmulsk_status maxmulsk_grouped_plans(const maxmulsk_gemm* problems, size_t n, maxmulsk_mode mode,
+                                       maxmulsk_plan* out);
 
 /* Grouped autotuner: shape class = (M rounded up to 64/128/256/512+, N, K).
  * Coordinate search over the candidate set, one class at a time (heaviest

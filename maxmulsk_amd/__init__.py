@@ -80,6 +80,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_grouped_info": (C.c_int, [_P, C.POINTER(_SIZE), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
     "maxmulsk_grouped_destroy": (None, [_P]),
     "maxmulsk_grouped_default_plan": (Plan, [_SIZE, _SIZE, _SIZE, C.c_int]),
+    "maxmulsk_grouped_plans": (C.c_int, [C.POINTER(Gemm), _SIZE, C.c_int, C.POINTER(Plan)]),
     "maxmulsk_grouped_autotune": (C.c_int, [C.POINTER(Gemm), _SIZE, C.c_int, C.c_int, C.c_int,
                                             C.POINTER(Plan), C.POINTER(C.c_float)]),
     "maxmulsk_num_candidates": (C.c_int, []),
@@ -346,6 +347,14 @@ def sgemm_grouped(problems: Sequence[tuple[Any, Any, Any, int, int, int]], mode:
 
 def grouped_default_plan(M: int, K: int, N: int, mode: int = MODE_PLAIN) -> tuple[int, int]:
     return lib().maxmulsk_grouped_default_plan(M, K, N, mode).as_tuple()
+
+
+def grouped_plans(problems, mode: int = MODE_PLAIN) -> list[tuple[int, int]]:
+    """Plans a grouped launch of these problems takes by default, one per problem."""
+    gemms = make_gemms(problems)
+    out = (Plan * max(len(problems), 1))()
+    check(lib().maxmulsk_grouped_plans(gemms, len(problems), mode, out))
+    return [out[i].as_tuple() for i in range(len(problems))]
 
 
 def grouped_autotune(problems, mode: int = MODE_PLAIN, max_classes: int = 0, reps: int = 3):

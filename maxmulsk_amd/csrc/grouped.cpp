@@ -372,6 +372,15 @@ extern "C" maxmulsk_plan maxmulsk_grouped_default_plan(size_t M, size_t K, size_
     return grouped_default(M, K, N, (int)mode);
 }
 
+extern "C" maxmulsk_status maxmulsk_grouped_plans(const maxmulsk_gemm* problems, size_t n, maxmulsk_mode mode,
+                                                  maxmulsk_plan* out) {
+    if (n > 0 && (!problems || !out)) return fail(MAXMULSK_ERR_UNSUPPORTED, "problems or out is null");
+    if ((int)mode < 0 || (int)mode > 2) return fail(MAXMULSK_ERR_UNSUPPORTED, "unknown mode");
+    const std::vector<maxmulsk_plan> p = batch_defaults(problems, n, (int)mode, true);
+    for (size_t i = 0; i < n; ++i) out[i] = p[i];
+    return MAXMULSK_OK;
+}
+
 extern "C" maxmulsk_status maxmulsk_sgemm_grouped(const maxmulsk_gemm* problems, size_t n,
                                                   const maxmulsk_options* opt) {
     const int mode = opt ? (int)opt->mode : 0;

@@ -120,6 +120,39 @@ def test_candidates_and_rule_plan():
         assert (t, s) in cands
 
 
+def _rule_split(K):
+    return min(16, max(1, -(-K // 4096)))
+
+
+def test_grouped_batch_plans():
+    """Default plans of a grouped launch (pure host logic, no device): a small
+    launch gets the per-problem rule (tile edge from N, ceil(K/4096) slices);
+    a launch of at least 1024 one-slice tiles runs unsliced, except a problem
+    one of whose tiles holds more than the launch's work per CU."""
+    none = lambda M, K, N: (None, None, None, M, K, N)
+    small = [none(37, 1536, 2112), none(300, 999, 1025), none(512, 16384, 512), none(5, 0, 7), none(0, 9, 9),
+             none(200, 64, 96)]
+    for mode in (0, 1, 2):
+        got = mm.grouped_plans(small, mode)
+        for (_, _, _, M, K, N), p in zip(small, got):
+            if M == 0 or N == 0 or K == 0:
+                assert p == (0, 0)
+            else:
+                assert p == mm.grouped_default_plan(M, K, N, mode) == (256 if N >= 256 else 128 if N >= 128 else 64,
+                                                                       _rule_split(K))
+    # the 256-problem workload: 8510 one-slice tiles, no tile longer than the work per CU
+    rng = np.random.default_rng(2024)
+    c4 = [none(int(rng.integers(1, 513)), int(rng.choice([512, 1536, 7168, 16384])), int(rng.choice([512, 2112, 4096, 7168])))
+          for _ in range(256)]
+    for mode in (0, 1):
+        assert all(p == (256, 1) for p in mm.grouped_plans(c4, mode))
+    # 600 short problems (1200 tiles) and one whose two tiles are each longer than the work per CU
+    mixed = [none(64, 8192, 512)] * 600 + [none(64, 1 << 20, 512)]
+    got = mm.grouped_plans(mixed, 1)
+    assert got[:600] == [(256, 1)] * 600 and got[600] == (256, 16)
+    assert mm.grouped_plans([], 0) == []
+
+
 def test_workspace_bytes():
     assert mm.workspace_bytes(64, 8192, 512, 0, (64, 16)) == 16 * 64 * 512 * 4
     # 256-wide tiles on more than 128 rows take the packed path: the workspace

@@ -48,7 +48,8 @@ def test_grouped_matches_oracle_and_standalone(mode):
     torch.cuda.synchronize()
     assert mm.sgemm_grouped(as_args(probs), mode=mode) == mm.OK, mm.last_error()
     torch.cuda.synchronize()
-    for (dA, dB, dC, M, K, N), (A, B) in zip(probs, host):
+    plans = mm.grouped_plans(as_args(probs), mode)  # what the launch ran
+    for (dA, dB, dC, M, K, N), (A, B), gplan in zip(probs, host, plans):
         assert dA.guards_intact() and dB.guards_intact() and dC.guards_intact()
         if M == 0 or N == 0:
             assert np.all(dC.numpy((-1,)) == np.float32(SENTINEL))
@@ -58,7 +59,8 @@ def test_grouped_matches_oracle_and_standalone(mode):
             assert np.all(Cg == 0.0)
             continue
         assert full_check(A, B, Cg)["bound_ratio"] <= 1.0, (M, K, N)
-        plan = standalone_plan(M, mm.grouped_default_plan(M, K, N, mode))
+        assert gplan == mm.grouped_default_plan(M, K, N, mode)  # a small launch: the per-problem plan
+        plan = standalone_plan(M, gplan)
         # same slices, same products per element, same reduce order (the
         # split-mode grouped kernels run 64-tile plans on 64 x 128 tiles with
         # eight waves: other wave tiles, same products)
