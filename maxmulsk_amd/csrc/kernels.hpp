@@ -138,6 +138,9 @@ struct GroupLaunch {
     unsigned int* counter;
 };
 hipError_t launch_grouped(int mode, const GroupLaunch& g, int grid, hipStream_t stream);
+// Plain-mode kernel of launch_grouped (its own translation unit) and its occupancy.
+hipError_t launch_grouped_plain(const GroupLaunch& g, int grid, hipStream_t stream);
+int grouped_plain_blocks_per_cu();
 // Largest tile geometry the persistent kernel serves.
 constexpr int kGroupedMaxCfg = 4;
 // The split-mode persistent kernels run 512-thread workgroups and every tile
