@@ -54,11 +54,11 @@ static hipError_t launch_one(const GemmParams& p, hipStream_t stream) {
     return hipGetLastError();
 }
 
-template <int MODE>
+template <int MODE, int BK>
 static hipError_t launch_deep(const GemmParams& p, hipStream_t stream) {
     constexpr TileCfg c = kTileCfgs[0];
     const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
-    hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, kDeepBK[MODE], c.wgm, c.wgn>), dim3((unsigned)grid),
+    hipLaunchKernelGGL((gemm_kernel<MODE, c.bm, c.bn, BK, c.wgm, c.wgn>), dim3((unsigned)grid),
                        dim3(64 * c.wgm * c.wgn), 0, stream, p);
     return hipGetLastError();
 }
@@ -69,7 +69,9 @@ static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) 
         case 0:
             if constexpr (kDeepBK[MODE] > 0) {
                 const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
-                if (grid > 0 && grid <= kDeepMaxGrid) return launch_deep<MODE>(p, stream);
+                if (grid > 0 && grid <= kDeepMaxGrid) return launch_deep<MODE, kDeepBK[MODE]>(p, stream);
+                if constexpr (kDeep2BK[MODE] > 0)
+                    if (grid > 0 && grid <= 2 * kDeepMaxGrid) return launch_deep<MODE, kDeep2BK[MODE]>(p, stream);
             }
             return launch_one<MODE, 0>(p, stream);
         case 1: return launch_one<MODE, 1>(p, stream);

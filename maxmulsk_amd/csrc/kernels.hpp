@@ -40,6 +40,12 @@ constexpr TileCfg kTileCfgs[kNumTileCfgs] = {
 // three BF16 terms make the stages large).
 constexpr int kDeepBK[3] = {64, 128, 0};
 constexpr int kDeepMaxGrid = 256;
+// Grids of up to two workgroups per CU (kDeepMaxGrid < grid <= 2 * kDeepMaxGrid)
+// run geometry 0 with k-blocks of kDeep2BK[mode]: 64 KiB of LDS per workgroup,
+// so the whole grid is resident at once. Measured: 67 x 513 x 8191 (plan 64,16,
+// 288 workgroups) plain 41.8 -> 37.0 us and split3 28.9 -> 26.6 us; 1000^3
+// plain 35.3 -> 32.1 us. Same slice boundaries and products as above.
+constexpr int kDeep2BK[3] = {64, 64, 0};
 
 // Direct launches of geometries 3 (128 x 128) and 4 (128 x 256) in the split
 // modes run eight waves (2 x 4) instead of the table's four: twice the waves

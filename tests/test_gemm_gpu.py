@@ -54,16 +54,19 @@ def test_every_split_count_reduces_exactly(mode):
 
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("shape,plan", [((64, 8192, 512), (64, 16)), ((128, 32768, 512), (64, 16)),
-                                        ((100, 999, 300), (64, 4)), ((512, 4096, 512), (64, 4))],
-                         ids=["64x512x8192", "128x512x32768", "100x300x999", "512x512x4096"])
+                                        ((100, 999, 300), (64, 4)), ((512, 4096, 512), (64, 4)),
+                                        ((67, 8191, 513), (64, 16)), ((1000, 1000, 1000), (64, 2))],
+                         ids=["64x512x8192", "128x512x32768", "100x300x999", "512x512x4096",
+                              "67x513x8191", "1000x1000x1000"])
 def test_deep_small_grid_tile_matches_regular_tile(mode, shape, plan):
-    """A 64-tile launch of at most 256 workgroups runs the deep-k-block variant;
-    the grouped kernel always runs the regular 32-deep one. Same slices, same
-    summation order: the two must agree bit for bit on random data."""
+    """A 64-tile launch of at most 256 workgroups runs the deep-k-block variant,
+    one of at most 512 the 64-deep one; the grouped kernel always runs its own
+    k-block depth. Same slices, same summation order: the two must agree bit for
+    bit on random data."""
     torch = torch_cuda()
     M, K, N = shape
     tiles = -(-M // 64) * -(-N // 64)
-    assert tiles * plan[1] <= 256  # the deep variant is the one launched
+    assert tiles * plan[1] <= 512  # one of the deep variants is the one launched
     A = orc.uniform(7000 + M, (M, K))
     B = orc.uniform(8000 + N, (K, N))
     single = gpu_gemm(A, B, mode=mode, plan=plan)
