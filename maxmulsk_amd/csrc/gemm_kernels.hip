@@ -76,7 +76,17 @@ static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) 
             return launch_one<MODE, 0>(p, stream);
         case 1: return launch_one<MODE, 1>(p, stream);
         case 2: return launch_one<MODE, 2>(p, stream);
-        case 3: return launch_one<MODE, 3>(p, stream);
+        case 3:
+            if constexpr (kDeep128BK[MODE] > 0) {
+                const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
+                if (grid > 0 && grid <= kDeepMaxGrid) {
+                    constexpr int WGM = direct_wgm(MODE, 3), WGN = direct_wgn(MODE, 3);
+                    hipLaunchKernelGGL((gemm_kernel<MODE, 128, 128, kDeep128BK[MODE], WGM, WGN>), dim3((unsigned)grid),
+                                       dim3(64 * WGM * WGN), 0, stream, p);
+                    return hipGetLastError();
+                }
+            }
+            return launch_one<MODE, 3>(p, stream);
         case 4: return launch_one<MODE, 4>(p, stream);
         default: return hipErrorInvalidValue;  // geometry 5 runs on the packed path
     }

@@ -46,6 +46,12 @@ constexpr int kDeepMaxGrid = 256;
 // 288 workgroups) plain 41.8 -> 37.0 us and split3 28.9 -> 26.6 us; 1000^3
 // plain 35.3 -> 32.1 us. Same slice boundaries and products as above.
 constexpr int kDeep2BK[3] = {64, 64, 0};
+// Deep variant of geometry 3 (128 x 128, eight waves in the split modes) for
+// grids of at most kDeepMaxGrid workgroups: k-blocks of kDeep128BK[mode] (128 KiB
+// of LDS in split3). Measured in split3: 2048^3 (plan 128,1) 67.2 -> 60.4 us,
+// 2048 x 1024 x 2048 42.5 -> 39.1 us, 512 x 4096 x 4096 69.5 -> 64.2 us. Plain
+// measured within +-1% and keeps 32; split6 would need 192 KiB.
+constexpr int kDeep128BK[3] = {0, 64, 0};
 
 // Direct launches of geometries 3 (128 x 128) and 4 (128 x 256) in the split
 // modes run eight waves (2 x 4) instead of the table's four: twice the waves

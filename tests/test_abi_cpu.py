@@ -350,15 +350,16 @@ def test_lds_access_patterns_are_bank_conflict_free(cfg, mode):
     # eight-wave tiles of the split modes: direct 128 x 128, grouped 64 x 128 .. 128 x 256
     *[(m, bm, bn, 32, 512, 2) for m in (1, 2) for bm, bn in ((128, 128), (64, 128), (64, 256), (128, 256))],
     (0, 64, 64, 64, 256, 4), (1, 64, 64, 128, 256, 4),     # deep 64 x 64 for small grids
+    (1, 64, 64, 64, 256, 4), (1, 128, 128, 64, 512, 2),    # 64-deep 64 x 64 (two per CU), split3 128 x 128
     # plain tiles of the grouped kernel: 16-deep k-blocks
     *[(0, bm, bn, 16, 256, 4) for bm, bn in ((64, 64), (64, 128), (64, 256), (128, 128), (128, 256))],
 ])
 def test_lds_patterns_of_the_launch_variants(mode, bm, bn, bk, threads, b_cols):
     """Layouts launched besides the table's: the eight-wave tiles of the split
     modes (two-column B units, as every eight-wave tile; the direct 128 x 128
-    tile and every tile of the grouped kernel), the deep 64 x 64 tiles (plain
-    k-blocks of 64, split3 of 128) and the plain tiles of the grouped kernel
-    (k-blocks of 16)."""
+    tile and every tile of the grouped kernel), the deep tiles (64 x 64: plain
+    k-blocks of 64, split3 of 128 or 64; split3 128 x 128: 64) and the plain
+    tiles of the grouped kernel (k-blocks of 16)."""
     _check_tile_lds_patterns(mode, bm, bn, bk, threads, b_cols)
 
 

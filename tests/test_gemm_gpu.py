@@ -82,13 +82,16 @@ def test_deep_small_grid_tile_matches_regular_tile(mode, shape, plan):
 
 @pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("shape,plan", [((300, 999, 1025), (128, 2)), ((256, 4096, 512), (128, 1)),
-                                        ((1000, 640, 700), (128, 4)), ((100, 2000, 1536), (256, 2))],
-                         ids=["300x1025x999", "256x512x4096", "1000x700x640", "100x1536x2000"])
+                                        ((1000, 640, 700), (128, 4)), ((100, 2000, 1536), (256, 2)),
+                                        ((1024, 1536, 1024), (128, 1)), ((512, 4095, 1000), (128, 2))],
+                         ids=["300x1025x999", "256x512x4096", "1000x700x640", "100x1536x2000",
+                              "1024x1024x1536", "512x1000x4095"])
 def test_eight_wave_128_tile_matches_grouped_tile(mode, shape, plan):
     """Direct split-mode launches of the 128 x 128 and 128 x 256 tiles run
-    eight waves, and the grouped kernel runs its own eight-wave tiles. Both
-    compute the same products in the same order as the four-wave layout, so
-    the results must agree bit for bit."""
+    eight waves (split3 128 x 128 launches of at most 256 workgroups with
+    64-deep k-blocks), and the grouped kernel runs its own eight-wave tiles.
+    Both compute the same products in the same order as the four-wave layout,
+    so the results must agree bit for bit."""
     torch = torch_cuda()
     M, K, N = shape
     A = orc.uniform(9000 + M, (M, K))
