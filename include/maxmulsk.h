@@ -292,7 +292,9 @@ int maxmulsk_lds_chunk_offset(int mode, int op, int rows, int bk, int row, int c
 /* Staging geometry of tile geometry cfg (0..maxmulsk_num_tile_geometries()-1)
  * in a mode, as one-call launches run it: out[0..5] = BM, BN, BK, threads per
  * workgroup, columns per B staging unit, waves (eight for the 128-row tiles of
- * the split modes). Returns 0, or -1 for an unknown cfg or mode. */
+ * the split modes). Launches of at most 512 workgroups may run the 64 x 64 and
+ * split3 128 x 128 tiles with deeper k-blocks than BK (same K slices). Returns
+ * 0, or -1 for an unknown cfg or mode. */
 int maxmulsk_num_tile_geometries(void);
 int maxmulsk_tile_info(int cfg, int mode, int* out);
 
