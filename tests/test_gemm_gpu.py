@@ -206,16 +206,16 @@ def test_v3_v4_and_host_pointers_agree():
         assert full_check(A, B, Cx)["bound_ratio"] <= 1.0
 
 
-def test_host_operands_larger_than_the_staging_chunks():
-    """Host operands that span several 8 MiB staging chunks, with ragged
-    sizes, in pageable and pinned host memory, mixed with device operands:
-    every combination gives the device-pointer result bit for bit and leaves
-    the device operands and C's guards intact."""
+@pytest.mark.parametrize("plan", [(128, 1), (256, 2)])
+def test_host_operands_larger_than_the_staging_chunks(plan):
+    """Host operands that span several 8 MiB staging chunks, with ragged sizes,
+    in pageable and pinned host memory, mixed with device operands: every
+    combination gives the device-pointer result bit for bit and leaves the
+    device operands and C's guards intact."""
     torch = torch_cuda()
-    M, N, K = 1500, 1900, 1700  # A 10.2 MB, B 12.9 MB, C 11.4 MB
+    M, N, K = 2600, 1300, 1700  # A 17.7 MB, B 8.8 MB, C 13.5 MB
     A = orc.uniform(71, (M, K))
     B = orc.uniform(72, (K, N))
-    plan = (128, 1)
     dA, dB = Guarded(M * K, A), Guarded(K * N, B)
     dC = Guarded(M * N, SENTINEL)
     assert mm.sgemm_ex(dA.ptr, dB.ptr, dC.ptr, M, K, N, mode=1, plan=plan) == mm.OK, mm.last_error()
@@ -227,7 +227,8 @@ def test_host_operands_larger_than_the_staging_chunks():
     pC = torch.empty((M, N), dtype=torch.float32).pin_memory()
     hC = np.empty((M, N), np.float32)
     cases = [("pageable A, B, C", A, B, hC), ("pinned A, pageable B, pinned C", pA, B, pC),
-             ("pageable A, device B, pageable C", A, dB.ptr, hC), ("device A, pinned B, device C", dA.ptr, pB, dC)]
+             ("pageable A, device B, pageable C", A, dB.ptr, hC), ("pageable A, B, device C", A, B, dC),
+             ("device A, pinned B, device C", dA.ptr, pB, dC)]
     for name, a, b, c in cases:
         if c is dC:
             dC.view.fill_(SENTINEL)
