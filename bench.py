@@ -122,7 +122,9 @@ def main() -> int:
             plan, src = mm.choose_plan(M, K, N, mode)
             assert src == 0, "MAXMULSK_PLAN did not parse as 'tile,split'"
             return plan
-        _, best = mm.autotune(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, mode, 3, 8)
+        # 5 repetitions per candidate and 15 per leader: the settings the
+        # selector's table was collected with
+        _, best = mm.autotune(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, mode, 5, 15)
         plan, src = mm.choose_plan(M, K, N, mode)
         assert src == 1 and plan == best, (plan, src, best)
         return plan
