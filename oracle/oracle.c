@@ -11,9 +11,9 @@
  *     FP64 value of the 3- and 6-product split GEMM (no reference counterpart);
  *   - the reference's plan rules: SMEKernels1x4V4C::choose
  *     (sme/v4/sme-1x4-v4c-dispatch.cpp:17-22), tuning::select_from_table for the
- *     1x4 kernel (sme/support/gemm_tuning.cpp:75,91-94,138-151), the v5.1 fixed
- *     plan (sme/v5/sme-1x4-fusedpack-v51.cpp:744-747), and NB::classify /
- *     capacity / counts (sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:410-466).
+ *     1x4 kernel (sme/support/gemm_tuning.cpp:75,91-94,138-151), and
+ *     NB::classify / capacity / counts
+ *     (sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:410-466).
  * Pinning of each function is documented in oracle/oracle.py.
  */
 #include <math.h>
@@ -141,12 +141,6 @@ void oracle_v3_select_from_table(size_t M, size_t K, size_t N, size_t* out) {
     } else if (M >= 2048) {
         out[0] = 1024; out[1] = 512; out[2] = 2048; out[3] = 1;
     }
-}
-
-/* SMEKernels1x4FusedPackV51::choose: one fixed plan {Mc, Nc, Kc}. */
-void oracle_v51_choose(size_t M, size_t K, size_t N, size_t* out) {
-    (void)M; (void)K; (void)N;
-    out[0] = 64; out[1] = 1024; out[2] = 1024;
 }
 
 /* NB::classify: 0 Native, 1 Unsupported (empty), 2 BadBlocking. */

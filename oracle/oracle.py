@@ -20,14 +20,18 @@ What it restates, and what pins each function (tests/test_oracle.py):
   trip of all 65536 BF16 bit patterns, ties-to-even cases built by hand, the
   representation bounds |x - sum(terms)| <= 2^-16|x| (2 terms) and 2^-24|x|
   (3 terms) on every exponent, and exactness on inputs that are already BF16.
-* ``v4c_choose`` / ``v3_select_from_table`` / ``v51_choose`` / ``nb_classify``
-  / ``nb_capacity`` / ``nb_counts`` — the reference's plan rules
+* ``v4c_choose`` / ``v3_select_from_table`` / ``nb_classify`` /
+  ``nb_capacity`` / ``nb_counts`` — the reference's plan rules
   (sme/v4/sme-1x4-v4c-dispatch.cpp:17-22, sme/support/gemm_tuning.cpp:75-151,
-  sme/v5/sme-1x4-fusedpack-v51.cpp:744-747,
-  sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:410-466). Pinned by the picks the
-  reference's own result files record per shape (v4c uses v4b exactly for
-  M >= 2048 with N > 512; the v3 headline rows use {1024,64,2048} except 2048^3
-  and 4096^3) and by the bad-blocking cases of check_nb.cpp:69-79.
+  sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:410-466). Pinned row by row
+  against the figures the reference's own result files print
+  (tests/golden/ref_plan_rows.npz, extracted by gen_ref_plan_rows.py):
+  a_packs, b_packs, mk_calls and both buffer sizes on all 7,862 rows of
+  nb_sweep.csv + nb_sweep_ext.csv; the v4c pick and its buffers on the 35
+  rows of v4c3.csv; the v3 pick on the 35 rows of final5.csv and the 66
+  MaxMulSK rows of headline_run*.csv / tuned_run*.csv (ir_outer included);
+  every field of nb_counts against a walk of the v4 loop nest itself
+  (ncblock.cpp:483-595); and the bad-blocking cases of check_nb.cpp:69-79.
 * ``uniform`` / ``log_uniform`` — the repo's own seeded generator
   (splitmix64 -> 24-bit or 53-bit fraction), pinned by committed golden values.
 """
@@ -67,7 +71,6 @@ def lib():
             "oracle_split_gemm_f64": [P, P, P, S, S, S, C.c_int],
             "oracle_v4c_choose": [S, S, S, P],
             "oracle_v3_select_from_table": [S, S, S, P],
-            "oracle_v51_choose": [S, S, S, P],
             "oracle_nb_capacity": [S, S, S, S, S, S, P],
             "oracle_nb_counts": [S, S, S, S, S, S, P],
         }
@@ -255,12 +258,6 @@ def v4c_choose(M: int, K: int, N: int) -> tuple[int, int, int]:
 def v3_select_from_table(M: int, K: int, N: int) -> tuple[int, int, int, int]:
     o = _sz(4)
     lib().oracle_v3_select_from_table(M, K, N, o)
-    return tuple(o)
-
-
-def v51_choose(M: int, K: int, N: int) -> tuple[int, int, int]:
-    o = _sz(3)
-    lib().oracle_v51_choose(M, K, N, o)
     return tuple(o)
 
 

@@ -188,32 +188,125 @@ def test_split_gemm_numpy_vs_c_and_exact_cases():
     assert np.max(np.abs(partial - C64) / ab) > 2.0 ** -12
 
 
-def test_reference_plan_rules():
-    # v4c: v4b exactly when M >= 2048 and N > 512 (sme-1x4-v4c-dispatch.cpp:17-22)
-    assert orc.v4c_choose(4096, 4096, 4096) == (32, 1024, 2048)
-    assert orc.v4c_choose(2048, 2048, 2048) == (32, 1024, 2048)
-    assert orc.v4c_choose(4096, 1536, 24576) == (32, 1024, 2048)
-    assert orc.v4c_choose(4096, 4096, 256) == (16, 512, 2048)   # N <= 512
-    for M, K, N in [(256, 256, 256), (1024, 1024, 1024), (64, 8192, 512), (128, 32768, 512),
-                    (64, 1536, 24576), (1023, 999, 1025), (67, 8191, 513)]:
-        assert orc.v4c_choose(M, K, N) == (16, 512, 2048)
-    # v3 table: squares >= 2048 only (gemm_tuning.cpp:91-94)
-    assert orc.v3_select_from_table(4096, 4096, 4096) == (2048, 512, 4096, 1)
-    assert orc.v3_select_from_table(2048, 2048, 2048) == (1024, 512, 2048, 1)
-    assert orc.v3_select_from_table(1024, 1024, 1024) == (1024, 64, 2048, 0)
-    assert orc.v3_select_from_table(64, 8192, 512) == (1024, 64, 2048, 0)
-    assert orc.v51_choose(1, 2, 3) == (64, 1024, 1024)
-    # classify: bad blockings of check_nb.cpp:69-79 and empty shapes
+REF = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_plan_rows.npz"))
+
+
+def _mib3(nbytes: int) -> float:
+    """Bytes as the reference's sweeps print them: MiB with three decimals."""
+    return float(f"{nbytes / 2.0 ** 20:.3f}")
+
+
+def walk_nb_loops(M, K, N, Mc, Nc, Kc):
+    """Counts events by walking the v4 loop nest itself, independently of the
+    closed forms: run_multiplication's clamps
+    (sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:638-654), then run_streaming's
+    Kc -> Nc -> Mc -> 64-column -> 16-row loops (:483-595): B panels packed on
+    mm == 0, A panels packed on nr == 0, one micro-kernel call per 16 x 64 tile.
+    Returns (kc_slices, nc_blocks, mc_blocks, a_panel_packs, b_panel_packs,
+    microkernel_calls) for microtile-aligned shapes (M % 16 == 0, N % 64 == 0),
+    the case NB::counts describes (:463-465)."""
+    Kc = min(Kc, K)
+    Nc = min(Nc, (N + 63) // 64 * 64)
+    Mc = min(Mc, (M + 15) // 16 * 16)
+    kcs = ncs = mcs = a = b = mk = 0
+    for kk in range(0, K, Kc):
+        kcs += 1
+        for nn in range(0, N, Nc):
+            if kk == 0:
+                ncs += 1
+            ncl = min(Nc, N - nn)
+            for mm in range(0, M, Mc):
+                if kk == 0 and nn == 0:
+                    mcs += 1
+                mcl = min(Mc, M - mm)
+                for nr in range(0, ncl, 64):
+                    if mm == 0:
+                        b += 1
+                    for ir in range(0, mcl, 16):
+                        if nr == 0:
+                            a += 1
+                        mk += 1
+    return kcs, ncs, mcs, a, b, mk
+
+
+def test_nb_counts_and_capacity_on_every_sweep_row():
+    """All 7,862 native rows of nb_sweep.csv + nb_sweep_ext.csv: a_packs,
+    b_packs, mk_calls and both packed-buffer sizes as the reference printed
+    them, for every blocking of the sweep."""
+    rows, mib = REF["nb"], REF["nb_mib"]
+    assert len(rows) == 7862 and rows[:, 9].all()
+    for r, (pa, pb) in zip(rows.tolist(), mib.tolist()):
+        M, K, N, Mc, Nc, Kc, a_packs, b_packs, mk = r[:9]
+        assert orc.nb_classify(M, K, N, Mc, Nc, Kc) == 0, r
+        c = orc.nb_counts(M, K, N, Mc, Nc, Kc)
+        assert c[3:] == (a_packs, b_packs, mk), (r, c)
+        ca, cb = orc.nb_capacity(M, K, N, Mc, Nc, Kc)
+        assert (_mib3(ca), _mib3(cb)) == (pa, pb), (r, ca, cb)
+
+
+def test_nb_counts_all_six_fields_against_the_loop_nest():
+    """Every field of nb_counts, including the block counts the sweep files do
+    not print, against the loop walker, on every sweep shape x a spread of the
+    sweep's blockings (including Kc > K and Nc > N clamps)."""
+    shapes = sorted({tuple(r[:3]) for r in REF["nb"].tolist()})
+    assert len(shapes) == 35
+    blockings = [(16, 64, 256), (16, 512, 2048), (32, 1024, 2048), (64, 256, 1024),
+                 (128, 4096, 16384), (128, 64, 512), (48, 192, 320)]
+    for (M, K, N) in shapes:
+        for (Mc, Nc, Kc) in blockings:
+            assert orc.nb_counts(M, K, N, Mc, Nc, Kc) == walk_nb_loops(M, K, N, Mc, Nc, Kc), (M, K, N, Mc, Nc, Kc)
+
+
+def test_loop_walker_catches_planted_mistakes():
+    """The walker distinguishes the fields a swapped or dropped term would
+    confuse: a_panel_packs (per Kc slice and Nc block) from b_panel_packs (per
+    Kc slice only), and nc_blocks from mc_blocks."""
+    c = walk_nb_loops(128, 4096, 1024, 32, 256, 1024)
+    assert c == (4, 4, 4, 4 * 4 * 8, 4 * 16, 4 * 16 * 8)
+    swapped = c[:3] + (c[4], c[3]) + c[5:]
+    assert swapped != c
+
+
+def test_v4c_choose_on_every_v4c3_row():
+    """v4c3.csv: the profile (v4a/v4b) and blocking the v4c rule picked for each
+    of its 35 shapes, and that blocking's packed-buffer sizes."""
+    rows, mib = REF["v4c"], REF["v4c_mib"]
+    assert len(rows) == 35 and set(rows[:, 3].tolist()) == {0, 1}
+    for r, (pa, pb) in zip(rows.tolist(), mib.tolist()):
+        M, K, N, prof, Mc, Nc, Kc = r
+        assert orc.v4c_choose(M, K, N) == (Mc, Nc, Kc), r
+        assert (Mc, Nc, Kc) == ((32, 1024, 2048) if prof else (16, 512, 2048))
+        ca, cb = orc.nb_capacity(M, K, N, Mc, Nc, Kc)
+        assert (_mib3(ca), _mib3(cb)) == (pa, pb), r
+
+
+def test_v3_table_and_v4_capacities_on_every_final5_row():
+    """final5.csv: the v3 tuning table's (M_tile, N_tile, Kc) for 35 shapes, and
+    NB::capacity of both v4c blockings (v4a and v4b) for each."""
+    rows, mib = REF["final5"], REF["final5_mib"]
+    assert len(rows) == 35
+    for r, m in zip(rows.tolist(), mib.tolist()):
+        M, K, N, mt, nt, kc = r
+        assert orc.v3_select_from_table(M, K, N)[:3] == (mt, nt, kc), r
+        got = orc.nb_capacity(M, K, N, 16, 512, 2048) + orc.nb_capacity(M, K, N, 32, 1024, 2048)
+        assert tuple(_mib3(x) for x in got) == tuple(m), (r, got)
+
+
+def test_v3_table_on_every_headline_row():
+    """headline_run*.csv and tuned_run*.csv: the blocking column, including the
+    ir_outer ("swap") flag, on all 66 MaxMulSK rows."""
+    rows = REF["headline"]
+    assert len(rows) == 66
+    for r in rows.tolist():
+        assert orc.v3_select_from_table(*r[:3]) == tuple(r[3:]), r
+
+
+def test_nb_classify_bad_blockings():
+    # bad blockings of check_nb.cpp:69-79 (refused, C untouched) and empty shapes
+    # (ncblock.cpp:411); every sweep row above is classified native.
     assert orc.nb_classify(256, 256, 256, 24, 256, 2048) == 2
     assert orc.nb_classify(256, 256, 256, 16, 100, 2048) == 2
     assert orc.nb_classify(256, 256, 256, 16, 256, 100) == 2
     assert orc.nb_classify(0, 256, 256, 16, 256, 2048) == 1
-    assert orc.nb_classify(256, 256, 256, 16, 256, 2048) == 0
-    # capacity at 4096^3 with v4b: packed_A 32*2048*4, packed_B 1024*2048*4
-    assert orc.nb_capacity(4096, 4096, 4096, 32, 1024, 2048) == (32 * 2048 * 4, 1024 * 2048 * 4)
-    # 256^3 with v4a: Nc clamps to 256, Kc to 256
-    assert orc.nb_capacity(256, 256, 256, 16, 512, 2048) == (16 * 256 * 4, 256 * 256 * 4)
-    # counts for 64x8192(K)x512 with v4a: 4 Kc slices, 1 Nc block, 4 Mc blocks
-    c = orc.nb_counts(64, 8192, 512, 16, 512, 2048)
-    assert c[:3] == (4, 1, 4)
-    assert c[5] == 4 * (512 // 64) * (64 // 16)
+    assert orc.nb_classify(256, 0, 256, 16, 256, 2048) == 1
+    assert orc.nb_classify(256, 256, 0, 16, 256, 2048) == 1
