@@ -2,9 +2,13 @@
 through the C ABI, and checks against the oracle."""
 from __future__ import annotations
 
+import json
+import os
+
 import numpy as np
 
 import maxmulsk_amd as mm
+from oracle import gpu_oracle as go
 from oracle import oracle as orc
 
 SENTINEL = 3.75
@@ -65,6 +69,39 @@ def gpu_gemm(A: np.ndarray, B: np.ndarray, mode: int = 0, plan=None, entry: str 
     if check_guards:
         assert dA.guards_intact() and dB.guards_intact() and dC.guards_intact(), "guard band written"
     return dC.numpy((M, N))
+
+
+def gpu_gemm_checked(A: np.ndarray, B: np.ndarray, mode: int = 0, plan=None, entry: str = "ex"):
+    """gpu_gemm, then every element of C checked against FP64 A*B by the GPU
+    oracle on the same device buffers. Returns (C, errors)."""
+    torch = torch_cuda()
+    M, K = A.shape
+    N = B.shape[1]
+    dA, dB = Guarded(M * K, A), Guarded(K * N, B)
+    dC = Guarded(M * N, SENTINEL)
+    torch.cuda.synchronize()
+    if entry == "ex":
+        st = mm.sgemm_ex(dA.ptr, dB.ptr, dC.ptr, M, K, N, mode=mode, plan=plan)
+    elif entry == "v4":
+        st = mm.sgemm_v4(dA.ptr, dB.ptr, dC.ptr, M, K, N)
+    else:
+        raise ValueError(entry)
+    assert st == mm.OK, f"status {st}: {mm.last_error()}"
+    torch.cuda.synchronize()
+    assert dA.guards_intact() and dB.guards_intact() and dC.guards_intact(), "guard band written"
+    e = go.check(dA.ptr, dB.ptr, dC.ptr, M, K, N)
+    return dC.numpy((M, N)), e
+
+
+def report(kind: str, **fields) -> None:
+    """Appends one JSON line to $MAXMULSK_REPORT_DIR/<kind>.jsonl when that
+    variable is set (accuracy tables collected from a test run)."""
+    d = os.environ.get("MAXMULSK_REPORT_DIR")
+    if not d:
+        return
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, f"{kind}.jsonl"), "a") as f:
+        f.write(json.dumps(fields) + "\n")
 
 
 def full_check(A, B, Cg, mode: int = 0, K_override=None):

@@ -6,7 +6,7 @@ import pytest
 
 import maxmulsk_amd as mm
 from oracle import oracle as orc
-from tests.gpu_util import SENTINEL, Guarded, cols_check, full_check, gpu_gemm, rows_check, torch_cuda
+from tests.gpu_util import SENTINEL, Guarded, full_check, gpu_gemm, gpu_gemm_checked, report, torch_cuda
 
 pytestmark = pytest.mark.gpu
 
@@ -118,19 +118,21 @@ PLAIN_SHAPES = [
 
 @pytest.mark.parametrize("mnk", PLAIN_SHAPES, ids=lambda s: "x".join(map(str, s)))
 def test_plain_within_fp32_bound(mnk):
+    """Every element of C against FP64 (the GPU oracle), through the
+    reference's own entry point; small shapes also against the CPU oracle."""
     M, N, K = mnk
     A = orc.uniform(11, (M, K))
     B = orc.uniform(12, (K, N))
-    Cg = gpu_gemm(A, B, mode=0, entry="v4")
+    Cg, e = gpu_gemm_checked(A, B, mode=0, entry="v4")
     assert not np.any(Cg == np.float32(SENTINEL))
-    if M * N * K <= 2 ** 31:
-        e = full_check(A, B, Cg)
-    else:
-        e = rows_check(A, B, Cg)
-        e2 = cols_check(A, B, Cg)
-        assert e2["bound_ratio"] <= 1.0, e2
+    if M * N * K <= 2 ** 30:
+        ec = full_check(A, B, Cg)
+        assert ec["bound_ratio"] <= 1.0, ec
+        assert abs(ec["bound_ratio"] - e["bound_ratio"]) <= 1e-5 * ec["bound_ratio"], (e, ec)
     assert e["nonfinite"] == 0
     assert e["bound_ratio"] <= 1.0, e
+    plan, _ = mm.choose_plan(M, K, N, 0)
+    report("accuracy", shape_mnk=[M, N, K], mode="plain", dist="uniform", plan=list(plan), **e)
     print(f"plain {M}x{N}x{K}: {e}")
 
 
@@ -148,20 +150,25 @@ SPLIT_SHAPES = [(2048, 2048, 2048), (4096, 4096, 4096), (64, 7168, 16384), (4096
                 (1023, 1025, 999), (67, 513, 8191)]
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 @pytest.mark.parametrize("dist", ["uniform", "logu"])
 @pytest.mark.parametrize("mnk", SPLIT_SHAPES, ids=lambda s: "x".join(map(str, s)))
 def test_split_within_fp32_bound(mode, dist, mnk):
+    """Every element of C against FP64 (the GPU oracle) on the split-precision
+    workload shapes, both distributions, beside plain FP32 MFMA (mode 0) on
+    the same inputs."""
     M, N, K = mnk
     gen = orc.uniform if dist == "uniform" else orc.log_uniform
     A = gen(31, (M, K))
     B = gen(32, (K, N))
-    Cg = gpu_gemm(A, B, mode=mode)
-    if M * N * K <= 2 ** 31:
-        e = full_check(A, B, Cg)
-    else:
-        e = rows_check(A, B, Cg, nrows=24)
+    Cg, e = gpu_gemm_checked(A, B, mode=mode)
+    if M * N * K <= 2 ** 30:
+        ec = full_check(A, B, Cg)
+        assert abs(ec["bound_ratio"] - e["bound_ratio"]) <= 1e-5 * ec["bound_ratio"], (e, ec)
     assert e["nonfinite"] == 0
+    plan, _ = mm.choose_plan(M, K, N, mode)
+    report("accuracy", shape_mnk=[M, N, K], mode=["plain", "split3", "split6"][mode], dist=dist,
+           plan=list(plan), **e)
     print(f"mode {mode} {dist} {M}x{N}x{K}: {e}")
     # split6 is FP32-level on any input; split3 drops lo*lo (<= 2^-16 |a||b| per
     # product), inside the K*u32 bound for these K on both distributions

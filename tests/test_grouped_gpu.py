@@ -199,3 +199,29 @@ def test_config4_style_batch_sampled():
             d = np.abs(Ch[ii, jj].astype(np.float64) - ref)
             assert np.all(d <= K * orc.U32 * absr), (mode, M, K, N)
         gp.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_config4_batch_every_element(mode):
+    """The 256-problem batch bench.py times (same seed, same aliasing of one
+    input pool): every element of every problem's C against FP64 through the
+    GPU oracle, within K * 2^-24 * |A||B|, after two graph replays."""
+    torch = torch_cuda()
+    import bench
+    from maxmulsk_amd import inputs
+    from oracle import gpu_oracle as go
+
+    specs, pool, outs, probs = bench.config4_batch(torch, inputs)
+    for c in outs:
+        c.fill_(SENTINEL)
+    gp = mm.GroupedPlan(probs, mode=mode)
+    gp.launch()
+    gp.launch()
+    torch.cuda.synchronize()
+    gp.close()
+    worst = 0.0
+    for (a, b, c, M, K, N) in probs:
+        e = go.check(a, b, c, M, K, N)
+        assert e["nonfinite"] == 0 and e["bound_ratio"] <= 1.0, (mode, M, K, N, e)
+        worst = max(worst, e["bound_ratio"])
+    print(f"mode {mode}: 256 problems, worst bound ratio {worst:.3g}")
