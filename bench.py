@@ -166,9 +166,9 @@ def main() -> int:
             if mode_name not in args.modes.split(","):
                 continue
             plan = tune(dA, dB, dC, M, K, N, mode)
-            ws = workspace(M, K, N, mode, plan)
-            # a plan call: planner (cache tier) + launch, as maxmulsk_sgemm_v4 runs it
-            ms, wall = timed(make_call(dA, dB, dC, M, K, N, mode, None, ws), args.steps, args.warmup)
+            # a plan call: planner (cache tier) + launch on the library's pooled
+            # workspace, as maxmulsk_sgemm_v4 runs it
+            ms, wall = timed(make_call(dA, dB, dC, M, K, N, mode), args.steps, args.warmup)
             e = err_fp64(dA, dB, dC, K)
             tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
             source = "MAXMULSK_PLAN" if os.environ.get("MAXMULSK_PLAN") else "autotune cache"
@@ -236,8 +236,7 @@ def main() -> int:
             row = {"shape_mnk": [M, N, K]}
             for mode_name, mode in (("plain", 0), ("split3", 1)):
                 plan = tune(dA, dB, dC, M, K, N, mode)
-                ws = workspace(M, K, N, mode, plan)
-                ms, _ = timed(make_call(dA, dB, dC, M, K, N, mode, None, ws), max(args.steps, 20), args.warmup)
+                ms, _ = timed(make_call(dA, dB, dC, M, K, N, mode), max(args.steps, 20), args.warmup)
                 e = err_fp64(dA, dB, dC, K)
                 flops = 2.0 * M * N * K
                 alg_bytes = 4.0 * (M * K + K * N + M * N)

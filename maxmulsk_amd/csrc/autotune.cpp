@@ -5,12 +5,14 @@
 //            median of the per-pass averages;
 //   stage 2: the leaders (best three, anything within 3% of the best, and the
 //            shape rule's plan), longer, rotated again;
-//   gate:    a candidate whose result disagrees with the first valid one on a
-//            strided sample of C is excluded from every ranking.
+//   gate:    a candidate whose strided sample of C is not within the mode's
+//            bound of FP64 dot products of the same outputs is excluded from
+//            every ranking.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -55,10 +57,12 @@ struct Tuner {
 
 // Strided sample of C: at most 4096 values at a fixed stride, gathered on the
 // device into `buf` and copied back in one piece.
+constexpr size_t kSampleMax = 4096;
+size_t sample_stride(size_t M, size_t N) { return std::max<size_t>(1, M * N / std::min<size_t>(M * N, kSampleMax)); }
+size_t sample_count(size_t M, size_t N) { return std::min<size_t>(M * N, kSampleMax); }
+
 std::vector<float> sample_c(const float* C, size_t M, size_t N, float* buf, hipStream_t s) {
-    const size_t total = M * N;
-    const size_t count = std::min<size_t>(total, 4096);
-    const size_t stride = std::max<size_t>(1, total / count);
+    const size_t count = sample_count(M, N), stride = sample_stride(M, N);
     std::vector<float> out(count);
     if (launch_gather_strided(C, stride, count, buf, s) != hipSuccess ||
         hipMemcpyAsync(out.data(), buf, count * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -69,16 +73,37 @@ std::vector<float> sample_c(const float* C, size_t M, size_t N, float* buf, hipS
     return out;
 }
 
-bool agree(const std::vector<float>& a, const std::vector<float>& ref) {
-    if (a.size() != ref.size() || a.empty()) return false;
-    double num = 0, den = 0;
-    for (size_t i = 0; i < a.size(); ++i) {
-        if (!std::isfinite(a[i])) return false;
-        const double d = (double)a[i] - ref[i];
-        num += d * d;
-        den += (double)ref[i] * ref[i];
+// Allowed error at a sampled output, as a multiple of its |A||B|: the FP32
+// bound K * 2^-24 for each product the mode accumulates, plus what the split
+// modes leave out by construction (split3: lo*lo and the two-term
+// representation error, under 3 * 2^-16 per product; split6: under 3 * 2^-24).
+// A candidate that computes a different product (a wrong tile, slice or
+// reduce) misses by orders of magnitude more.
+double gate_coef(int mode, size_t K) {
+    const int P = mode == 0 ? 1 : (mode == 1 ? 3 : 6);
+    const double drop = mode == 0 ? 0.0 : (mode == 1 ? std::ldexp(1.0, -14) : std::ldexp(1.0, -22));
+    return P * (double)K * std::ldexp(1.0, -24) + drop;
+}
+
+bool within_gate(const std::vector<float>& c, const std::vector<double>& ref, double coef) {
+    const size_t n = ref.size() / 2;  // values, then |A||B|
+    if (c.size() != n || n == 0) return false;
+    for (size_t i = 0; i < n; ++i) {
+        if (!std::isfinite(c[i])) return false;
+        if (std::fabs((double)c[i] - ref[i]) > coef * ref[n + i]) return false;
     }
-    return den == 0 ? num == 0 : std::sqrt(num / den) <= 1e-4;
+    return true;
+}
+
+// Test hook: MAXMULSK_TEST_GATE_REJECT=<i> makes candidate i's sample read as
+// wrong (a NaN in place of its first value), so tests can check that a wrong
+// candidate is excluded whatever its position.
+int gate_reject_hook() {
+    const char* v = std::getenv("MAXMULSK_TEST_GATE_REJECT");
+    if (!v || !*v) return -1;
+    char* end = nullptr;
+    const long i = std::strtol(v, &end, 10);
+    return end != v ? (int)i : -1;
 }
 
 }  // namespace
@@ -119,7 +144,8 @@ extern "C" maxmulsk_status maxmulsk_autotune(const float* A, const float* B, flo
     if ((!pl.tune_stream && hipStreamCreateWithFlags(&pl.tune_stream, hipStreamNonBlocking) != hipSuccess) ||
         (!pl.tune_e0 && hipEventCreate(&pl.tune_e0) != hipSuccess) ||
         (!pl.tune_e1 && hipEventCreate(&pl.tune_e1) != hipSuccess) ||
-        (!pl.tune_sample && hipMalloc(&pl.tune_sample, 4096 * sizeof(float)) != hipSuccess)) {
+        (!pl.tune_sample && device_malloc(&pl.tune_sample, kSampleMax * sizeof(float)) != hipSuccess) ||
+        (!pl.tune_ref && device_malloc(&pl.tune_ref, 2 * kSampleMax * sizeof(double)) != hipSuccess)) {
         (void)hipGetLastError();
         return fail(MAXMULSK_ERR_DEVICE, "stream/event creation failed");
     }
@@ -128,8 +154,22 @@ extern "C" maxmulsk_status maxmulsk_autotune(const float* A, const float* B, flo
     (void)hipDeviceSynchronize();
 
     std::vector<float> stage1(kNumCandidates, -1.f), final_t(kNumCandidates, -1.f);
-    std::vector<float> ref;
-    // Correctness gate: first valid run defines the reference sample.
+    // Correctness gate (the reference's sweep excludes a blocking whose
+    // checksum differs from the v3 result, bench_nb_sweep.cpp:116-122): every
+    // candidate's strided sample of C is checked against FP64 dot products of
+    // the same outputs, within the mode's bound (gate_coef).
+    const size_t count = sample_count(M, N);
+    std::vector<double> ref(2 * count);
+    if (launch_sample_dot_f64(A, B, K, N, sample_stride(M, N), count, pl.tune_ref, pl.tune_ref + count, t.s) !=
+            hipSuccess ||
+        hipMemcpyAsync(ref.data(), pl.tune_ref, 2 * count * sizeof(double), hipMemcpyDeviceToHost, t.s) !=
+            hipSuccess ||
+        hipStreamSynchronize(t.s) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MAXMULSK_ERR_DEVICE, "reference sample failed");
+    }
+    const double coef = gate_coef((int)mode, K);
+    const int reject = gate_reject_hook();
     for (int i = 0; i < kNumCandidates; ++i) {
         if (!valid[i]) continue;
         if (t.run(res[i]) != hipSuccess) {
@@ -138,14 +178,8 @@ extern "C" maxmulsk_status maxmulsk_autotune(const float* A, const float* B, flo
             continue;
         }
         std::vector<float> smp = sample_c(C, M, N, pl.tune_sample, t.s);
-        if (ref.empty()) {
-            bool finite = !smp.empty();
-            for (float v : smp) finite = finite && std::isfinite(v);
-            if (finite) ref = smp;
-            else valid[i] = false;
-        } else if (!agree(smp, ref)) {
-            valid[i] = false;
-        }
+        if (i == reject && !smp.empty()) smp[0] = std::nanf("");
+        valid[i] = within_gate(smp, ref, coef);
     }
 
     // Stage 1: rotated order, 2 passes.

@@ -57,7 +57,8 @@ class CopyWorkers {
 
     // Slice i of the current copy (4 KiB-aligned boundaries).
     void part(int i) const {
-        const size_t per = (n_ / parts_ + 4095) & ~size_t(4095);
+        // ceil(n / parts) rounded up to 4 KiB: the slices cover all n bytes
+        const size_t per = ((n_ + parts_ - 1) / parts_ + 4095) & ~size_t(4095);
         const size_t b = std::min(n_, per * (size_t)i), e = std::min(n_, b + per);
         if (e > b) std::memcpy(dst_ + b, src_ + b, e - b);
     }

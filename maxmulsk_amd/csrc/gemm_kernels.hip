@@ -148,3 +148,41 @@ hipError_t launch_gather_strided(const float* src, size_t stride, size_t count, 
 }
 
 }  // namespace mmsk
+
+namespace mmsk {
+
+// One wave per sampled output: lanes take k = lane, lane + 64, ... and the
+// partial sums are combined by a fixed butterfly, all in FP64.
+__global__ __launch_bounds__(256) void sample_dot_f64_kernel(const float* __restrict__ A, const float* __restrict__ B,
+                                                             size_t K, size_t N, size_t stride, size_t count,
+                                                             double* __restrict__ ref, double* __restrict__ absref) {
+    const size_t i = (size_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
+    if (i >= count) return;
+    const size_t idx = i * stride, r = idx / N, c = idx % N;
+    const float* a = A + r * K;
+    double s = 0.0, t = 0.0;
+    for (size_t k = lane; k < K; k += 64) {
+        const double x = (double)a[k], y = (double)B[k * N + c];
+        s = fma(x, y, s);
+        t = fma(fabs(x), fabs(y), t);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o);
+        t += __shfl_xor(t, o);
+    }
+    if (lane == 0) {
+        ref[i] = s;
+        absref[i] = t;
+    }
+}
+
+hipError_t launch_sample_dot_f64(const float* A, const float* B, size_t K, size_t N, size_t stride,
+                                 size_t count, double* ref, double* absref, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(sample_dot_f64_kernel, dim3((unsigned)((count + 3) / 4)), dim3(256), 0, stream, A, B, K, N,
+                       stride, count, ref, absref);
+    return hipGetLastError();
+}
+
+}  // namespace mmsk

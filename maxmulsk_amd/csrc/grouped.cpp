@@ -138,7 +138,7 @@ hipError_t enqueue(const maxmulsk_grouped_plan_s* p, hipStream_t s) {
 template <class T>
 bool upload(T** dst, const std::vector<T>& v) {
     if (v.empty()) return true;
-    if (hipMalloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T)) != hipSuccess) return false;
+    if (device_malloc(reinterpret_cast<void**>(dst), v.size() * sizeof(T)) != hipSuccess) return false;
     return hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) == hipSuccess;
 }
 
@@ -281,11 +281,11 @@ maxmulsk_status build(const maxmulsk_gemm* problems, size_t n, int mode, const m
 
     bool ok = true;
     count(kCntAllocation);  // the plan's device buffers
-    if (ws_bytes) ok = hipMalloc(&p->d_ws, ws_bytes) == hipSuccess;
+    if (ws_bytes) ok = device_malloc(&p->d_ws, ws_bytes) == hipSuccess;
     if (ok)
         for (size_t j = 0; j < gp.size(); ++j)
             if (gp[j].splits > 1) gp[j].P = reinterpret_cast<float*>(static_cast<char*>(p->d_ws) + ws_off[j]);
-    ok = ok && hipMalloc(reinterpret_cast<void**>(&p->d_counter), sizeof(unsigned int)) == hipSuccess;
+    ok = ok && device_malloc(reinterpret_cast<void**>(&p->d_counter), sizeof(unsigned int)) == hipSuccess;
     ok = ok && upload(&p->d_problems, gp) && upload(&p->d_item_problem, item_problem) &&
          upload(&p->d_block_problem, block_problem);
     if (!ok) {

@@ -175,6 +175,12 @@ hipError_t launch_fill_zero_f32(float* p, size_t n, hipStream_t stream);
 // dst[i] = src[i * stride] for i < count.
 hipError_t launch_gather_strided(const float* src, size_t stride, size_t count, float* dst,
                                  hipStream_t stream);
+// FP64 reference of the same strided sample of C = A*B (A M x K, B K x N,
+// row-major): for i < count, flat index i * stride = r * N + c,
+// ref[i] = sum_k A[r][k] B[k][c] and absref[i] = sum_k |A[r][k]| |B[k][c]|,
+// accumulated in FP64 (the autotuner's correctness gate).
+hipError_t launch_sample_dot_f64(const float* A, const float* B, size_t K, size_t N, size_t stride,
+                                 size_t count, double* ref, double* absref, hipStream_t stream);
 
 // ---- MLP (learned selector) ---------------------------------------------------
 constexpr int kMlpIn = 4;

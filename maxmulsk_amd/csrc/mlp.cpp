@@ -33,6 +33,9 @@ struct Selector {
     std::vector<float> host;                  // installed weights (empty: none)
     std::map<int, float*> dev;                // per-device copy of `host`
     std::map<int, float*> io;                 // per-device 4 inputs + 64 outputs
+    // Per-device non-blocking stream of the inference kernel: a plan lookup
+    // never joins the legacy default stream or a stream being captured.
+    std::map<int, hipStream_t> stream;
     std::map<std::tuple<unsigned long long, unsigned long long, unsigned long long, int>, maxmulsk_plan> memo;
     unsigned long long generation = 0;
 };
@@ -100,12 +103,14 @@ void autoload_locked() {
 }
 
 // Device copy of the installed weights on device d (caller holds g_sel.mu).
-float* device_weights_locked(int d) {
+float* device_weights_locked(int d, hipStream_t s) {
     auto it = g_sel.dev.find(d);
     if (it != g_sel.dev.end()) return it->second;
     float* p = nullptr;
     const size_t bytes = g_sel.host.size() * sizeof(float);
-    if (hipMalloc(&p, bytes) != hipSuccess || hipMemcpy(p, g_sel.host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    if (device_malloc(&p, bytes) != hipSuccess ||
+        hipMemcpyAsync(p, g_sel.host.data(), bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
         (void)hipGetLastError();
         if (p) (void)hipFree(p);
         return nullptr;
@@ -114,11 +119,23 @@ float* device_weights_locked(int d) {
     return p;
 }
 
+hipStream_t device_stream_locked(int d) {
+    auto it = g_sel.stream.find(d);
+    if (it != g_sel.stream.end()) return it->second;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_sel.stream[d] = s;
+    return s;
+}
+
 float* device_io_locked(int d) {
     auto it = g_sel.io.find(d);
     if (it != g_sel.io.end()) return it->second;
     float* p = nullptr;
-    if (hipMalloc(&p, 128 * sizeof(float)) != hipSuccess) {
+    if (device_malloc(&p, 128 * sizeof(float)) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
     }
@@ -137,13 +154,14 @@ void features(size_t M, size_t K, size_t N, int mode, float* x) {
 maxmulsk_status predict_locked(size_t M, size_t K, size_t N, int mode, float* pred, maxmulsk_plan* best) {
     int d = 0;
     if (!current_device(&d) || !device_usable(d)) return fail(MAXMULSK_ERR_UNSUPPORTED, "no gfx950 device");
-    float* w = device_weights_locked(d);
+    hipStream_t s = device_stream_locked(d);
+    if (!s) return fail(MAXMULSK_ERR_DEVICE, "selector stream creation failed");
+    float* w = device_weights_locked(d, s);
     float* io = device_io_locked(d);
     if (!w || !io) return fail(MAXMULSK_ERR_ALLOC, "selector device buffers");
     float x[4];
     features(M, K, N, mode, x);
     float y[kOut];
-    hipStream_t s = nullptr;
     hipError_t e = hipMemcpyAsync(io, x, sizeof x, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = launch_mlp_infer(w, io, kOut, io + 64, s);
     if (e == hipSuccess) e = hipMemcpyAsync(y, io + 64, sizeof y, hipMemcpyDeviceToHost, s);
@@ -221,7 +239,7 @@ extern "C" maxmulsk_status maxmulsk_mlp_backward(const float* params, const floa
         if (sc.first) (void)hipFree(sc.first);
         sc.first = nullptr;
         sc.second = 0;
-        if (hipMalloc(&sc.first, need * sizeof(float)) != hipSuccess) {
+        if (device_malloc(&sc.first, need * sizeof(float)) != hipSuccess) {
             (void)hipGetLastError();
             return fail(MAXMULSK_ERR_ALLOC, "mlp scratch allocation failed");
         }

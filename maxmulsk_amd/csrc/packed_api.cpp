@@ -57,7 +57,7 @@ extern "C" maxmulsk_status maxmulsk_pack(const float* X, size_t rows, size_t K, 
 
     DeviceGuard guard(dev);
     void* buf = nullptr;
-    if (hipMalloc(&buf, bytes) != hipSuccess) {
+    if (device_malloc(&buf, bytes) != hipSuccess) {
         (void)hipGetLastError();
         return fail(MAXMULSK_ERR_ALLOC, "allocation of %zu packed bytes failed", bytes);
     }
@@ -148,6 +148,7 @@ extern "C" maxmulsk_status maxmulsk_sgemm_packed(maxmulsk_packed A, maxmulsk_pac
         count(kCntReduce);
     }
     if (e == hipSuccess && pooled) e = pool_workspace_release(pl, s);
+    lock.unlock();  // the wait below needs nothing from the pools
     if (e == hipSuccess && !async) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
         (void)hipGetLastError();

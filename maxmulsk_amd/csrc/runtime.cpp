@@ -4,6 +4,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -118,6 +119,18 @@ DeviceGuard::~DeviceGuard() {
 
 DevicePools& pools(int dev) { return g_pools[dev]; }
 
+hipError_t device_malloc_bytes(void** p, size_t bytes) {
+    if (const char* lim = std::getenv("MAXMULSK_TEST_ALLOC_LIMIT")) {
+        char* end = nullptr;
+        const unsigned long long limit = std::strtoull(lim, &end, 10);
+        if (end != lim && bytes > limit) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+    }
+    return hipMalloc(p, bytes);
+}
+
 bool grow(void** buf, size_t* cap, size_t bytes) {
     if (*cap >= bytes && *buf) return true;
     const size_t want = (bytes + (2u << 20) - 1) / (2u << 20) * (2u << 20);
@@ -129,7 +142,7 @@ bool grow(void** buf, size_t* cap, size_t bytes) {
     }
     void* p = nullptr;
     Scope scope("maxmulsk/allocation");
-    if (hipMalloc(&p, want) != hipSuccess) {
+    if (device_malloc(&p, want) != hipSuccess) {
         (void)hipGetLastError();
         return false;
     }
@@ -378,6 +391,9 @@ maxmulsk_status run_gemm(const GemmCall& c) {
     if (staged && c.async) return fail(MAXMULSK_ERR_UNSUPPORTED, "async calls need device pointers");
     if (staged && c.ws_required) return fail(MAXMULSK_ERR_UNSUPPORTED, "the workspace entry point needs device pointers");
 
+    // The operands' device is current from here on: the selector tier of the
+    // planner runs its inference kernel there.
+    DeviceGuard guard(dev);
     maxmulsk_plan plan = c.plan;
     bool resolved;
     {
@@ -401,7 +417,11 @@ maxmulsk_status run_gemm(const GemmCall& c) {
                         c.ws_bytes, r.ws_bytes);
     }
 
-    DeviceGuard guard(dev);
+    // The pool lock covers what the call takes from the device pools: the
+    // workspace (until the launch and its release event are enqueued) and, for
+    // host operands, the staging and bounce buffers (until C is back on the
+    // host). A synchronous call on device memory waits for its stream without
+    // it, so callers on different streams overlap on the GPU.
     DevicePools& pl = pools(dev);
     std::unique_lock<std::mutex> lock(pl.mu);
 
@@ -455,6 +475,13 @@ maxmulsk_status run_gemm(const GemmCall& c) {
         e = stage_out(pl, c.C, C, c.M * c.N * sizeof(float), c.stream);
         count(kCntWriteback);
     }
+    if (e == hipSuccess && staged && c_dev) {
+        // Device C with staged operands: the staging buffer stays in use until
+        // the GEMM has run.
+        Scope scope("maxmulsk/sync");
+        e = hipStreamSynchronize(c.stream);
+    }
+    lock.unlock();
     if (e == hipSuccess && !c.async) {
         Scope scope("maxmulsk/sync");
         e = hipStreamSynchronize(c.stream);

@@ -59,8 +59,18 @@ struct DevicePools {
     hipStream_t tune_stream = nullptr;
     hipEvent_t tune_e0 = nullptr, tune_e1 = nullptr;
     float* tune_sample = nullptr;  // 4096 floats: gathered sample of C
+    double* tune_ref = nullptr;    // 2 x 4096 doubles: FP64 value and |A||B| of the sample
 };
 DevicePools& pools(int dev);
+
+// hipMalloc for every device buffer the library owns. For tests of the
+// allocation-failure paths, MAXMULSK_TEST_ALLOC_LIMIT=<bytes> makes any larger
+// request fail as hipMalloc would (hipErrorOutOfMemory), without allocating.
+hipError_t device_malloc_bytes(void** p, size_t bytes);
+template <class T>
+hipError_t device_malloc(T** p, size_t bytes) {
+    return device_malloc_bytes(reinterpret_cast<void**>(p), bytes);
+}
 
 // Grows a pool buffer to at least `bytes` (rounded up to 2 MiB). Waits for the
 // device before freeing the old buffer. Returns false on allocation failure.

@@ -23,7 +23,11 @@ int main(int argc, char** argv) {
         callers.emplace_back([&, t] {
             std::mt19937_64 rng(1000 + t);
             for (int it = 0; it < iters; ++it) {
-                const size_t n = 1 + rng() % (5u << 20);
+                // every fourth copy: a multiple of 16 KiB (above the parallel
+                // threshold) plus 1..3 bytes, where slices of n / parts rounded
+                // down to 4 KiB would miss the tail
+                const size_t n = (it % 4 == 3) ? (size_t)(64 + rng() % 256) * 16384 + 1 + it % 3
+                                               : 1 + rng() % (5u << 20);
                 std::vector<unsigned char> src(n), dst(n, 0);
                 for (size_t i = 0; i < n; i += 509) src[i] = (unsigned char)(rng() | 1);
                 src[n - 1] = 0x5a;

@@ -66,3 +66,42 @@ def test_grouped_autotune_small_batch():
     for (dA, dB, dC, M, K, N), (A, B) in zip(probs, host):
         assert full_check(A, B, dC.numpy((M, N)))["bound_ratio"] <= 1.0
     gp.close()
+
+
+@pytest.mark.parametrize("reject", [0, 7])
+def test_autotune_gate_excludes_a_wrong_candidate(reject, monkeypatch):
+    """The gate checks every candidate's sampled outputs against FP64 dot
+    products, not against another candidate: a candidate whose sample reads
+    wrong (test hook: a NaN in its sample) is excluded even when it is the
+    first one, and every other candidate still qualifies."""
+    torch = torch_cuda()
+    lib = mm.lib()
+    lib.maxmulsk_cache_clear()
+    M, K, N = 80, 2000, 520
+    A = orc.uniform(93, (M, K))
+    B = orc.uniform(94, (K, N))
+    dA, dB, dC = Guarded(M * K, A), Guarded(K * N, B), Guarded(M * N, SENTINEL)
+    torch.cuda.synchronize()
+    monkeypatch.setenv("MAXMULSK_TEST_GATE_REJECT", str(reject))
+    times, best = mm.autotune(dA.ptr, dB.ptr, dC.ptr, M, K, N, 1, 2, 3)
+    monkeypatch.delenv("MAXMULSK_TEST_GATE_REJECT")
+    assert times[reject] < 0, times
+    assert all(t > 0 for i, t in enumerate(times) if i != reject), times
+    assert best != mm.candidates()[reject]
+    lib.maxmulsk_cache_clear()
+
+
+def test_autotune_gate_accepts_split3_on_wide_dynamic_range():
+    """The gate's bound follows the mode: split3 on log-uniform inputs (2^-20..2^20)
+    stays eligible, with every candidate accepted."""
+    torch = torch_cuda()
+    lib = mm.lib()
+    lib.maxmulsk_cache_clear()
+    M, K, N = 130, 4000, 300
+    A = orc.log_uniform(95, (M, K))
+    B = orc.log_uniform(96, (K, N))
+    dA, dB, dC = Guarded(M * K, A), Guarded(K * N, B), Guarded(M * N, SENTINEL)
+    torch.cuda.synchronize()
+    times, best = mm.autotune(dA.ptr, dB.ptr, dC.ptr, M, K, N, 1, 2, 3)
+    assert all(t > 0 for t in times), times
+    lib.maxmulsk_cache_clear()

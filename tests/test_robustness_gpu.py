@@ -2,6 +2,7 @@
 bytes and 2^31 elements, a very long K, and concurrent callers on several
 streams sharing the library's per-device pools."""
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -159,3 +160,51 @@ def test_async_calls_alternating_streams_share_the_pool():
     torch.cuda.synchronize()
     for i, (A, B, dA, dB, dC) in enumerate(calls):
         assert full_check(A, B, dC.cpu().numpy())["bound_ratio"] <= 1.0, i
+
+
+def test_synchronous_calls_on_different_streams_overlap():
+    """A synchronous call waits for its own stream only: thread A's call sits
+    behind a bounded GPU sleep on stream A while thread B's call on stream B
+    starts later and returns first (the per-device pool lock is not held
+    across the wait). Both results stay correct."""
+    torch = torch_cuda()
+    M, N, K = 96, 640, 2048
+    A = orc.uniform(701, (M, K))
+    B = orc.uniform(702, (K, N))
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    cA = torch.full((M, N), SENTINEL, dtype=torch.float32, device="cuda")
+    cB = torch.full((M, N), SENTINEL, dtype=torch.float32, device="cuda")
+    sA, sB = torch.cuda.Stream(), torch.cuda.Stream()
+    plan = (64, 1)  # no split-K: neither call takes the pooled workspace
+    assert mm.workspace_bytes(M, K, N, 1, plan) == 0
+    # warm up: kernels loaded, selector and pools initialised
+    for s, c in ((sA, cA), (sB, cB)):
+        assert mm.sgemm_ex(dA.data_ptr(), dB.data_ptr(), c.data_ptr(), M, K, N, mode=1, plan=plan, stream=s) == mm.OK
+    torch.cuda.synchronize()
+    done, status = {}, {}
+    entered = threading.Event()
+
+    def call_a():
+        with torch.cuda.stream(sA):
+            torch.cuda._sleep(int(2e9))  # about a second of GPU time on stream A
+        entered.set()
+        status["a"] = mm.sgemm_ex(dA.data_ptr(), dB.data_ptr(), cA.data_ptr(), M, K, N, mode=1, plan=plan, stream=sA)
+        done["a"] = time.perf_counter()
+
+    def call_b():
+        entered.wait(30)
+        time.sleep(0.1)  # A is inside its call, waiting on stream A
+        status["b"] = mm.sgemm_ex(dA.data_ptr(), dB.data_ptr(), cB.data_ptr(), M, K, N, mode=1, plan=plan, stream=sB)
+        done["b"] = time.perf_counter()
+
+    ta, tb = threading.Thread(target=call_a), threading.Thread(target=call_b)
+    ta.start()
+    tb.start()
+    ta.join(60)
+    tb.join(60)
+    assert not ta.is_alive() and not tb.is_alive()
+    assert status == {"a": mm.OK, "b": mm.OK}, (status, mm.last_error())
+    assert done["b"] < done["a"], "B waited for A's stream"
+    torch.cuda.synchronize()
+    for c in (cA, cB):
+        assert full_check(A, B, c.cpu().numpy())["bound_ratio"] <= 1.0
