@@ -99,6 +99,24 @@ maxmulsk_status maxmulsk_sgemm_ws(const float* A, const float* B, float* C,
                                   maxmulsk_mode mode, const maxmulsk_plan* plan,
                                   void* ws, size_t ws_bytes, void* stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc): maxmulsk_sgemm_ws with the workspace taken from a static device
+ * array of MAXMULSK_NOALLOC_SCRATCH_BYTES that the library holds on every GPU
+ * it runs on until the process ends. The call allocates nothing (no hipMalloc,
+ * hipFree or allocating library routine); its planner uses the
+ * MAXMULSK_PLAN override, the autotuner cache and the shape rule, not the
+ * learned selector. A, B and C must be device memory; the call is synchronous
+ * on `stream`. A plan whose workspace (maxmulsk_workspace_bytes) exceeds the
+ * scratch returns MAXMULSK_ERR_ALLOC with C untouched. All calls on one GPU
+ * share its scratch: calls that overlap on one GPU (from several threads or
+ * streams) race and must be serialised by the caller. */
+#define MAXMULSK_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+maxmulsk_status maxmulsk_sgemm_noalloc(const float* A, const float* B, float* C,
+                                       size_t M, size_t K, size_t N,
+                                       maxmulsk_mode mode, const maxmulsk_plan* plan,
+                                       void* stream);
+
 /* Pack once, multiply many: an operand rewritten into the MFMA fragment
  * layout (split into its BF16 terms in the split modes), owned by the library.
  * Operand 0 is an A (rows = M, row-major M x K), operand 1 a B (rows = N,

@@ -69,6 +69,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_sgemm_ex": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.POINTER(Options)]),
     "maxmulsk_workspace_bytes": (C.c_size_t, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan)]),
     "maxmulsk_sgemm_ws": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), _P, _SIZE, _P]),
+    "maxmulsk_sgemm_noalloc": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), _P]),
     "maxmulsk_pack": (C.c_int, [_P, _SIZE, _SIZE, C.c_int, C.c_int, _P, C.POINTER(_P)]),
     "maxmulsk_sgemm_packed": (C.c_int, [_P, _P, _P, C.POINTER(Options)]),
     "maxmulsk_packed_destroy": (None, [_P]),
@@ -219,6 +220,16 @@ def sgemm_ws(A: Any, B: Any, Cm: Any, M: int, K: int, N: int, mode: int, plan: t
     p = C.byref(Plan(*plan)) if plan else None
     return lib().maxmulsk_sgemm_ws(_ptr(A), _ptr(B), _ptr(Cm), M, K, N, mode, p,
                                    _ptr(ws) if ws is not None else None, ws_bytes, _stream_ptr(stream))
+
+
+NOALLOC_SCRATCH_BYTES = 32 << 20  # MAXMULSK_NOALLOC_SCRATCH_BYTES
+
+
+def sgemm_noalloc(A: Any, B: Any, Cm: Any, M: int, K: int, N: int, mode: int = MODE_PLAIN,
+                  plan: tuple[int, int] | None = None, stream: Any = None) -> int:
+    """maxmulsk_sgemm_noalloc: workspace from the library's static device scratch."""
+    p = C.byref(Plan(*plan)) if plan else None
+    return lib().maxmulsk_sgemm_noalloc(_ptr(A), _ptr(B), _ptr(Cm), M, K, N, mode, p, _stream_ptr(stream))
 
 
 def choose_plan(M: int, K: int, N: int, mode: int = MODE_PLAIN) -> tuple[tuple[int, int], int]:

@@ -241,4 +241,10 @@ maxmulsk_plan choose_plan(size_t M, size_t K, size_t N, int mode, int* source) {
     return p;
 }
 
+maxmulsk_plan choose_plan_no_selector(size_t M, size_t K, size_t N, int mode) {
+    maxmulsk_plan p{};
+    if (env_plan(&p) || cache_get(M, K, N, mode, &p)) return p;
+    return rule_plan(M, K, N, mode);
+}
+
 }  // namespace mmsk

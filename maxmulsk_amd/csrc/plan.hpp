@@ -74,5 +74,7 @@ bool selector_pick(size_t M, size_t K, size_t N, int mode, maxmulsk_plan* out);
 
 // Full planner: tiers 0..3. source receives the tier used.
 maxmulsk_plan choose_plan(size_t M, size_t K, size_t N, int mode, int* source);
+// Tiers 0, 1 and 3 (override, autotuner cache, shape rule): pure host code.
+maxmulsk_plan choose_plan_no_selector(size_t M, size_t K, size_t N, int mode);
 
 }  // namespace mmsk

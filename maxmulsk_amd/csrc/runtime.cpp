@@ -402,7 +402,8 @@ maxmulsk_status run_gemm(const GemmCall& c) {
             if (c.rule_only) {
                 if (!env_plan(&plan)) plan = rule_plan(c.M, c.K, c.N, c.mode);
             } else {
-                plan = choose_plan(c.M, c.K, c.N, c.mode, nullptr);
+                plan = c.no_alloc ? choose_plan_no_selector(c.M, c.K, c.N, c.mode)
+                                  : choose_plan(c.M, c.K, c.N, c.mode, nullptr);
             }
         }
         resolved = resolve(c.M, c.K, c.N, c.mode, plan, &r);
@@ -411,6 +412,9 @@ maxmulsk_status run_gemm(const GemmCall& c) {
         return fail(MAXMULSK_ERR_UNSUPPORTED, "plan {%d,%d} rejected for %zux%zux%zu", plan.tile,
                     plan.split_k, c.M, c.K, c.N);
 
+    if (c.no_alloc && r.ws_bytes > c.ws_bytes)
+        return fail(MAXMULSK_ERR_ALLOC, "plan {%d,%d} for %zux%zux%zu needs %zu bytes of workspace, above the "
+                    "%zu-byte static scratch", plan.tile, plan.split_k, c.M, c.K, c.N, r.ws_bytes, c.ws_bytes);
     if (c.ws_required || c.ws) {
         if (r.ws_bytes > 0 && (!c.ws || c.ws_bytes < r.ws_bytes || ((uintptr_t)c.ws & 255) != 0))
             return fail(MAXMULSK_ERR_ALLOC, "workspace null, short (%zu < %zu) or misaligned",

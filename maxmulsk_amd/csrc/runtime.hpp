@@ -102,6 +102,10 @@ struct GemmCall {
     void* ws;             // caller workspace (nullptr: pool)
     size_t ws_bytes;
     bool ws_required;     // caller promised a workspace (the _ws entry point)
+    // Static-scratch entry point: the planner skips the selector tier (its
+    // first use on a device allocates), and a workspace above ws_bytes is
+    // refused as too large for the scratch.
+    bool no_alloc = false;
 };
 maxmulsk_status run_gemm(const GemmCall& c);
 
