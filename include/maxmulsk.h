@@ -58,7 +58,7 @@ typedef enum {
 } maxmulsk_mode;
 
 /* A launch plan: output tile edge (64, 128 or 256; the M edge shrinks to 64 or
- * 128 for narrow M) and the number of K slices (split-K, 1..16). {0, 0} asks
+ * 128 for narrow M) and the number of K slices (split-K, 1..32). {0, 0} asks
  * the planner. Analogue of the reference's Blocking {Mc, Nc, Kc}
  * (sme/v4/sme-1x4-kcout-ncblock-apack4za.hpp:30). */
 typedef struct {

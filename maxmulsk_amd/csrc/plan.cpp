@@ -14,7 +14,7 @@ namespace mmsk {
 
 maxmulsk_plan candidate(int i) {
     if (i < 0 || i >= kNumCandidates) return maxmulsk_plan{0, 0};
-    return maxmulsk_plan{kCandTiles[i / 5], kCandSplits[i % 5]};
+    return maxmulsk_plan{kCandTiles[i / kNumSplitCands], kCandSplits[i % kNumSplitCands]};
 }
 
 int candidate_index(maxmulsk_plan p) {

@@ -10,9 +10,10 @@
 namespace mmsk {
 
 constexpr int kCandTiles[3] = {64, 128, 256};
-constexpr int kCandSplits[5] = {1, 2, 4, 8, 16};
-constexpr int kNumCandidates = 15;  // index = tile_index * 5 + split_index
-constexpr int kMaxSplit = 16;
+constexpr int kCandSplits[6] = {1, 2, 4, 8, 16, 32};
+constexpr int kNumSplitCands = 6;
+constexpr int kNumCandidates = 3 * kNumSplitCands;  // index = tile_index * 6 + split_index
+constexpr int kMaxSplit = 32;
 constexpr int kNumCUs = 256;        // MI355X compute units
 
 // Candidate i of the fixed candidate set.
@@ -42,7 +43,7 @@ struct Resolved {
 
 // Counterpart of classify (sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:410-423)
 // and the clamps of run_multiplication (:638-654): rejects a plan whose tile is
-// not 64/128/256 or whose split is outside 1..16, then clamps the split to the
+// not 64/128/256 or whose split is outside 1..kMaxSplit, then clamps the split to the
 // shape (no empty slices). Shapes must be non-empty.
 // max_cfg caps the geometry (the grouped kernel serves geometries 0..4);
 // min_cfg raises it (the split-mode grouped kernel serves geometries 1..4).

@@ -16,8 +16,6 @@
 
 namespace mmsk {
 
-static_assert(kMaxSplit == 16, "sum_splits4 dispatches split counts 1..16");
-
 template <int S>
 __device__ __forceinline__ f32x4 sum_splits4_fixed(const float* __restrict__ P, long long stride) {
     f32x4 v[S];
@@ -29,27 +27,22 @@ __device__ __forceinline__ f32x4 sum_splits4_fixed(const float* __restrict__ P, 
     return s;
 }
 
-// Four consecutive values (P[0..3] of every slice, dword-aligned) summed over
-// `splits` slices, 1 <= splits <= 16. The switch is uniform across the wave.
-__device__ __forceinline__ f32x4 sum_splits4(const float* __restrict__ P, long long stride, int splits) {
-    switch (splits) {
-        case 1: return sum_splits4_fixed<1>(P, stride);
-        case 2: return sum_splits4_fixed<2>(P, stride);
-        case 3: return sum_splits4_fixed<3>(P, stride);
-        case 4: return sum_splits4_fixed<4>(P, stride);
-        case 5: return sum_splits4_fixed<5>(P, stride);
-        case 6: return sum_splits4_fixed<6>(P, stride);
-        case 7: return sum_splits4_fixed<7>(P, stride);
-        case 8: return sum_splits4_fixed<8>(P, stride);
-        case 9: return sum_splits4_fixed<9>(P, stride);
-        case 10: return sum_splits4_fixed<10>(P, stride);
-        case 11: return sum_splits4_fixed<11>(P, stride);
-        case 12: return sum_splits4_fixed<12>(P, stride);
-        case 13: return sum_splits4_fixed<13>(P, stride);
-        case 14: return sum_splits4_fixed<14>(P, stride);
-        case 15: return sum_splits4_fixed<15>(P, stride);
-        default: return sum_splits4_fixed<16>(P, stride);
+// Dispatch of a wave-uniform split count onto the unrolled bodies S .. kMaxSplit.
+template <int S>
+__device__ __forceinline__ f32x4 sum_splits4_from(const float* __restrict__ P, long long stride, int splits) {
+    if constexpr (S >= kMaxSplit) {
+        return sum_splits4_fixed<kMaxSplit>(P, stride);
+    } else {
+        if (splits == S) return sum_splits4_fixed<S>(P, stride);
+        return sum_splits4_from<S + 1>(P, stride, splits);
     }
+}
+
+// Four consecutive values (P[0..3] of every slice, dword-aligned) summed over
+// `splits` slices, 1 <= splits <= kMaxSplit. The split count is uniform across
+// the wave.
+__device__ __forceinline__ f32x4 sum_splits4(const float* __restrict__ P, long long stride, int splits) {
+    return sum_splits4_from<1>(P, stride, splits);
 }
 
 }  // namespace mmsk

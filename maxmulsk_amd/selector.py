@@ -26,7 +26,7 @@ import numpy as np
 
 import maxmulsk_amd as mm
 
-NCAND = 15
+NCAND = 18  # maxmulsk_num_candidates(): tile {64, 128, 256} x split-K {1, 2, 4, 8, 16, 32}
 FIXED_PLAN = (128, 1)  # one plan for every shape: the "no selector" baseline
 
 

@@ -98,7 +98,7 @@ def test_phase_counters_names_and_reset():
 def test_invalid_plan_rejected_before_anything_runs():
     A = np.ones((8, 8), np.float32)
     C = np.full((8, 8), 9.0, np.float32)
-    for plan in [(32, 1), (100, 1), (64, 0), (64, 17), (256, -1)]:
+    for plan in [(32, 1), (100, 1), (64, 0), (64, 33), (256, -1)]:
         assert mm.sgemm_ex(A, A, C, 8, 8, 8, plan=plan) == mm.ERR_UNSUPPORTED
         assert np.all(C == 9.0)
         assert mm.workspace_bytes(8, 8, 8, 0, plan) == 0
@@ -107,12 +107,14 @@ def test_invalid_plan_rejected_before_anything_runs():
 
 def test_candidates_and_rule_plan():
     cands = mm.candidates()
-    assert len(cands) == 15
-    assert cands == [(t, s) for t in (64, 128, 256) for s in (1, 2, 4, 8, 16)]
+    assert len(cands) == 18
+    from maxmulsk_amd import selector as sel
+    assert sel.NCAND == len(cands) == mm.lib().maxmulsk_num_candidates()
+    assert cands == [(t, s) for t in (64, 128, 256) for s in (1, 2, 4, 8, 16, 32)]
     assert mm.rule_plan(4096, 4096, 4096) == (256, 1)
-    # narrow M: small tiles, K split to fill the CUs
+    # narrow M: small tiles, K split to fill the CUs (8 tiles x 32 slices of 256)
     t, s = mm.rule_plan(64, 8192, 512)
-    assert t == 64 and s == 16
+    assert t == 64 and s == 32
     t, s = mm.rule_plan(64, 512, 32768)
     assert s >= 1 and t in (64, 128, 256)
     for shape in [(1, 1, 1), (1023, 999, 1025), (67, 8191, 513), (4096, 1536, 24576)]:
@@ -121,7 +123,7 @@ def test_candidates_and_rule_plan():
 
 
 def _rule_split(K):
-    return min(16, max(1, -(-K // 4096)))
+    return min(32, max(1, -(-K // 4096)))
 
 
 def test_grouped_batch_plans():
@@ -149,7 +151,7 @@ def test_grouped_batch_plans():
     # 600 short problems (1200 tiles) and one whose two tiles are each longer than the work per CU
     mixed = [none(64, 8192, 512)] * 600 + [none(64, 1 << 20, 512)]
     got = mm.grouped_plans(mixed, 1)
-    assert got[:600] == [(256, 1)] * 600 and got[600] == (256, 16)
+    assert got[:600] == [(256, 1)] * 600 and got[600] == (256, 32)
     assert mm.grouped_plans([], 0) == []
 
 

@@ -10,7 +10,7 @@ from tests.gpu_util import SENTINEL, Guarded, full_check, gpu_gemm, gpu_gemm_che
 
 pytestmark = pytest.mark.gpu
 
-ALL_PLANS = [(t, s) for t in (64, 128, 256) for s in (1, 2, 4, 8, 16)]
+ALL_PLANS = [(t, s) for t in (64, 128, 256) for s in (1, 2, 4, 8, 16, 32)]
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
@@ -34,20 +34,20 @@ def _up256(x):
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_every_split_count_reduces_exactly(mode):
-    """Plan (tile, 16) on a K of s k-blocks runs exactly s slices, so the reduce
-    sees every split count 1..16 (direct 64-row tiles: k-blocks of 32; packed
+    """Plan (tile, 32) on a K of s k-blocks runs exactly s slices, so the reduce
+    sees every split count 1..32 (direct 64-row tiles: k-blocks of 32; packed
     256 x 256 tiles: k-blocks of 16). Small integers keep every partial sum
     exact, so the result must equal the integer product bit for bit."""
-    for s in range(1, 17):
+    for s in range(1, 33):
         for M, N, tile, depth in ((67, 93, 64, 32), (300, 260, 256, 16)):
             K = depth * s
-            extra = mm.workspace_bytes(M, K, N, mode, (tile, 16)) - (
+            extra = mm.workspace_bytes(M, K, N, mode, (tile, 32)) - (
                 mm.workspace_bytes(M, K, N, mode, (tile, 1)))
             assert extra == (_up256(s * M * N * 4) if s > 1 else 0), (s, tile, extra)
             A = orc.small_ints(3000 + s, (M, K), 8)
             B = orc.small_ints(4000 + s, (K, N), 8)
             exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float32)
-            Cg = gpu_gemm(A, B, mode=mode, plan=(tile, 16))
+            Cg = gpu_gemm(A, B, mode=mode, plan=(tile, 32))
             bad = np.argwhere(Cg != exact)
             assert bad.size == 0, f"mode {mode} splits {s} tile {tile}: {len(bad)} wrong, first {bad[:4].tolist()}"
 

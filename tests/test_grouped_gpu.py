@@ -98,13 +98,13 @@ def test_grouped_tail_rows_match_standalone(mode):
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_grouped_reduce_every_split_count(mode):
-    """Sixteen problems whose K spans 1..16 k-blocks of 32 under plan (64, 16):
-    one grouped reduce serves every split count. Small-integer inputs make the
-    products exact."""
+    """Thirty-two problems whose K spans 1..32 k-blocks of 32 under plan
+    (64, 32): one grouped reduce serves every split count. Small-integer inputs
+    make the products exact."""
     torch = torch_cuda()
-    specs = [(40 + s, 32 * s, 70 + 3 * s) for s in range(1, 17)]
+    specs = [(40 + s, 32 * s, 70 + 3 * s) for s in range(1, 33)]
     probs, host = build(specs, 800, gen=lambda seed, shape: orc.small_ints(seed, shape, 8))
-    gp = mm.GroupedPlan(as_args(probs), mode=mode, plans=[(64, 16)] * len(specs))
+    gp = mm.GroupedPlan(as_args(probs), mode=mode, plans=[(64, 32)] * len(specs))
     gp.launch()
     torch.cuda.synchronize()
     gp.close()

@@ -12,7 +12,7 @@ from oracle import mlp_oracle as mo
 from tests.gpu_util import torch_cuda
 
 pytestmark = pytest.mark.gpu
-NOUT = 15
+NOUT = sel.NCAND  # one output per candidate plan
 
 
 def data(n, seed):
