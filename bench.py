@@ -1,10 +1,19 @@
 """Benchmark: TFLOP/s at FP32 accuracy, split-BF16 vs plain FP32 MFMA, on one MI355X.
 
 A step is one C = A*B call through the C ABI (maxmulsk_sgemm_ex, device
-pointers, inputs resident in HBM) at M = N = K = 4096 in the split-BF16
-3-product mode. The same run also times the plain FP32-MFMA path at 4096^3 and
-both paths on the narrow-M shapes (M x N x K), and reports each result's error
-against an FP64 product.
+pointers, inputs resident in HBM, the library's pooled workspace) at
+M = N = K = 4096 in the split-BF16 3-product mode. The same run also times the
+plain FP32-MFMA path and split6 at 4096^3, both paths on the narrow-M shapes
+(M x N x K), and the 256-problem grouped batch with its plans picked by the
+grouped autotuner, and reports each result's error against an FP64 product.
+
+Timing follows the reference's benchmark method (bench_final5.cpp:72-151): a
+warm-up, then rounds in rotated (Latin-square) order over the entrants of one
+shape, each round timing one entrant over `steps` back-to-back calls between
+two HIP events on the launch stream; a figure is the median over the rounds
+and its spread is (max - min) / median. The reference treats differences
+below about 5% as noise (docs/BENCHMARKS.md:2262-2280); the line carries that
+floor next to every spread.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -28,10 +37,16 @@ BF16_PEAK = 2516.6   # TFLOP/s, MI355X dense BF16 MFMA
 FP32_PEAK = 157.3    # TFLOP/s, MI355X FP32 MFMA
 HBM_PEAK = 8000.0    # GB/s
 NARROW = [(64, 512, 8192), (64, 24576, 1536), (64, 32768, 512), (128, 512, 32768), (64, 7168, 16384)]
+NOISE_FLOOR_PCT = 5.0  # docs/BENCHMARKS.md:2262-2280
+MIN_ROUND_MS = 5.0     # GPU time per timed round, at least
 
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spread_pct(xs: list[float]) -> float:
+    return 100.0 * (max(xs) - min(xs)) / float(np.median(xs))
 
 
 def main() -> int:
@@ -39,10 +54,13 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--rounds", type=int, default=5, help="rotated rounds per entrant (median reported)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only", default="", help="comma list of sections: main,narrow,grouped (default all)")
     ap.add_argument("--modes", default="plain,split3,split6", help="modes timed at 4096^3")
+    ap.add_argument("--grouped-classes", type=int, default=16,
+                    help="shape classes the grouped autotuner searches (heaviest first)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device copy (reference for traffic counters)")
     args = ap.parse_args()
@@ -67,13 +85,12 @@ def main() -> int:
     def upload(x):
         return torch.from_numpy(x).to(dev)
 
-    def make_call(dA, dB, dC, M, K, N, mode, plan=None, ws=None):
+    def make_call(dA, dB, dC, M, K, N, mode, plan=None):
         """One maxmulsk_sgemm_ex call with its arguments built once, as a C
         caller holds them: the timed loop then measures the library and the
         GPU, not the Python wrapper (which costs more than a small GEMM)."""
         opt = mm.Options(mode=mode, plan=mm.Plan(*(plan or (0, 0))), stream=sptr, async_=1,
-                         workspace=ws.data_ptr() if ws is not None else None,
-                         workspace_bytes=ws.numel() * 4 if ws is not None else 0)
+                         workspace=None, workspace_bytes=0)
         fn, args_ = mm.lib().maxmulsk_sgemm_ex, (dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N,
                                                   ctypes.byref(opt))
 
@@ -84,27 +101,37 @@ def main() -> int:
         call.opt = opt  # keeps the options alive with the closure
         return call
 
-    def timed(fn, steps, warmup):
-        for _ in range(warmup):
-            fn()
+    def block(fn, steps):
+        """ms per call over `steps` back-to-back calls, HIP events on the stream."""
         torch.cuda.synchronize()
         t0 = torch.cuda.Event(enable_timing=True)
         t1 = torch.cuda.Event(enable_timing=True)
-        wall = time.perf_counter()
         t0.record(stream)
         for _ in range(steps):
             fn()
         t1.record(stream)
         torch.cuda.synchronize()
-        wall = time.perf_counter() - wall
-        return t0.elapsed_time(t1) / steps, wall
+        return t0.elapsed_time(t1) / steps
 
-    def err_fp64(dA, dB, dC, K):
-        return errors_torch(dA, dB, dC, K)
-
-    def workspace(M, K, N, mode, plan):
-        nbytes = mm.workspace_bytes(M, K, N, mode, plan)
-        return torch.empty(max(nbytes // 4, 1) + 64, dtype=torch.float32, device=dev) if nbytes else None
+    def rounds(fns: dict, steps: int, offset: int = 0) -> dict:
+        """Warm-up, then args.rounds rounds in rotated order; per entrant the
+        list of per-round ms per call. A round runs at least `steps` calls
+        and at least MIN_ROUND_MS of GPU time (the reference sizes its samples
+        the same way, adaptive repetitions of at least 0.3 s per sample), so
+        short calls are not timed over a handful of microseconds."""
+        names = list(fns)
+        reps = {}
+        for n in names:
+            for _ in range(args.warmup):
+                fns[n]()
+            est = block(fns[n], steps)
+            reps[n] = max(steps, int(np.ceil(MIN_ROUND_MS / max(est, 1e-4))))
+        res = {n: [] for n in names}
+        for r in range(args.rounds):
+            for j in range(len(names)):
+                n = names[(j + r + offset) % len(names)]
+                res[n].append(block(fns[n], reps[n]))
+        return res
 
     def tune(dA, dB, dC, M, K, N, mode):
         """Fill the plan cache for this shape with the autotuner's pick, so the
@@ -134,24 +161,20 @@ def main() -> int:
 
     def packed_kernel_ms(dA, dB, dC, M, K, N, mode, split, steps):
         """Average duration of the packed GEMM launch alone: both operands
-        packed once, then `steps` launches between two events on the stream
-        the kernel runs on."""
+        packed once, then rounds of `steps` launches between two events on
+        the stream the kernel runs on (median of the rounds)."""
         pa = mm.Packed(dA.data_ptr(), M, K, 0, mode, stream=sptr)
         pb = mm.Packed(dB.data_ptr(), N, K, 1, mode, stream=sptr)
-        ws = workspace(M, K, N, mode, (256, split)) if split > 1 else None
-        ws_ptr = ws.data_ptr() if ws is not None else None
-        ws_bytes = ws.numel() * 4 if ws is not None else 0
 
         def run():
-            st = mm.sgemm_packed(pa, pb, dC.data_ptr(), split_k=split, stream=sptr, async_=True,
-                                 workspace=ws_ptr, workspace_bytes=ws_bytes)
+            st = mm.sgemm_packed(pa, pb, dC.data_ptr(), split_k=split, stream=sptr, async_=True)
             if st != mm.OK:
                 raise RuntimeError(f"sgemm_packed status {st}: {mm.last_error()}")
 
-        ms, _ = timed(run, steps, 3)
+        ms = rounds({"k": run}, steps)["k"]
         pa.close()
         pb.close()
-        return ms
+        return float(np.median(ms)), spread_pct(ms)
 
     out: dict = {}
     shapes = []
@@ -161,29 +184,33 @@ def main() -> int:
         log("generating 4096^3 inputs")
         dA, dB = upload(inputs.uniform(1, (M, K))), upload(inputs.uniform(2, (K, N)))
         dC = torch.empty((M, N), dtype=torch.float32, device=dev)
+        modes = [(n, m) for n, m in (("plain", 0), ("split3", 1), ("split6", 2)) if n in args.modes.split(",")]
+        plans = {n: tune(dA, dB, dC, M, K, N, m) for n, m in modes}
+        # a plan call: planner (cache tier) + launch on the library's pooled
+        # workspace, as maxmulsk_sgemm_v4 runs it
+        times = rounds({n: make_call(dA, dB, dC, M, K, N, m) for n, m in modes}, args.steps)
         res = {}
-        for mode_name, mode in (("plain", 0), ("split3", 1), ("split6", 2)):
-            if mode_name not in args.modes.split(","):
-                continue
-            plan = tune(dA, dB, dC, M, K, N, mode)
-            # a plan call: planner (cache tier) + launch on the library's pooled
-            # workspace, as maxmulsk_sgemm_v4 runs it
-            ms, wall = timed(make_call(dA, dB, dC, M, K, N, mode), args.steps, args.warmup)
-            e = err_fp64(dA, dB, dC, K)
+        for n, m in modes:
+            make_call(dA, dB, dC, M, K, N, m)()
+            e = errors_torch(dA, dB, dC, K)
+            ms = float(np.median(times[n]))
             tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
             source = "MAXMULSK_PLAN" if os.environ.get("MAXMULSK_PLAN") else "autotune cache"
-            r = {"tflops": tf, "ms": ms, "plan": list(plan), "plan_source": source, "errors": e}
-            if plan[0] == 256:
+            r = {"tflops": tf, "ms": ms, "round_ms": times[n], "spread_pct": spread_pct(times[n]),
+                 "plan": list(plans[n]), "plan_source": source, "errors": e}
+            if plans[n][0] == 256:
                 # the 256-wide path packs both operands on every call; time its
                 # GEMM kernel alone on pre-packed operands
-                kms = packed_kernel_ms(dA, dB, dC, M, K, N, mode, plan[1], args.steps)
+                kms, ksp = packed_kernel_ms(dA, dB, dC, M, K, N, m, plans[n][1], args.steps)
                 r["gemm_kernel_ms"] = kms
+                r["gemm_kernel_spread_pct"] = ksp
                 r["packed_operands_tflops"] = 2.0 * M * N * K / (kms * 1e-3) / 1e12
+                r["pack_share_of_call"] = 1.0 - kms / ms
             else:
                 r["gemm_kernel_ms"] = ms
-            res[mode_name] = r
-            log(f"4096^3 {mode_name}: {ms:.4f} ms  {tf:.1f} TFLOP/s  plan {plan}  "
-                f"kernel {r['gemm_kernel_ms']:.4f} ms  err {e}")
+            res[n] = r
+            log(f"4096^3 {n}: {ms:.4f} ms (spread {r['spread_pct']:.1f}%) {tf:.1f} TFLOP/s plan {plans[n]} "
+                f"kernel {r['gemm_kernel_ms']:.4f} ms err {e}")
         if "split3" in res:
             # The drop-in case: maxmulsk_sgemm_v4 on host (pageable) arrays, so
             # each call stages A and B to the device and C back over PCIe.
@@ -193,7 +220,7 @@ def main() -> int:
             try:
                 mm.check(mm.sgemm_v4(A_h, B_h, C_h, M, K, N))
                 ts = []
-                for _ in range(3):
+                for _ in range(5):
                     t = time.perf_counter()
                     mm.check(mm.sgemm_v4(A_h, B_h, C_h, M, K, N))
                     ts.append(time.perf_counter() - t)
@@ -201,6 +228,7 @@ def main() -> int:
                 del os.environ["MAXMULSK_MODE"]
             hms = float(np.median(ts)) * 1e3
             res["split3"]["host_pointer_ms"] = hms
+            res["split3"]["host_pointer_spread_pct"] = spread_pct(ts)
             res["split3"]["host_pointer_tflops"] = 2.0 * M * N * K / (hms * 1e-3) / 1e12
             log(f"4096^3 split3 through maxmulsk_sgemm_v4 on host arrays (PCIe staging): {hms:.2f} ms "
                 f"{res['split3']['host_pointer_tflops']:.1f} TFLOP/s")
@@ -212,15 +240,14 @@ def main() -> int:
         for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
             a, b = dA.to(dt), dB.to(dt)
             c = torch.empty((M, N), dtype=dt, device=dev)
-            vms, _ = timed(lambda: torch.matmul(a, b, out=c), args.steps, args.warmup)
-            vendor[name] = {"ms": vms, "tflops": 2.0 * M * N * K / (vms * 1e-3) / 1e12}
+            vt = rounds({name: lambda: torch.matmul(a, b, out=c)}, args.steps)[name]
+            vms = float(np.median(vt))
+            vendor[name] = {"ms": vms, "tflops": 2.0 * M * N * K / (vms * 1e-3) / 1e12, "spread_pct": spread_pct(vt)}
             log(f"4096^3 torch.matmul {name}: {vms:.4f} ms {vendor[name]['tflops']:.1f} TFLOP/s")
             del a, b, c
-        res["vendor_torch_matmul"] = vendor
         out["main"] = res
         # the headline: the fastest split mode that stays inside the FP32 bound
         ok = [m for m in ("split3", "split6") if m in res and res[m]["errors"]["within_fp32_bound"]]
-        vendor = res.pop("vendor_torch_matmul")
         head = max(ok, key=lambda m: res[m]["tflops"]) if ok else next(iter(res))
         out["headline_mode"] = head
         out["vendor"] = vendor
@@ -228,29 +255,31 @@ def main() -> int:
         torch.cuda.empty_cache()
 
     if "narrow" in only:
-        for (M, N, K) in NARROW:
-            A = inputs.uniform(3, (M, K))
-            B = inputs.uniform(4, (K, N))
-            dA, dB = upload(A), upload(B)
+        for i, (M, N, K) in enumerate(NARROW):
+            dA, dB = upload(inputs.uniform(3, (M, K))), upload(inputs.uniform(4, (K, N)))
             dC = torch.empty((M, N), dtype=torch.float32, device=dev)
             row = {"shape_mnk": [M, N, K]}
-            for mode_name, mode in (("plain", 0), ("split3", 1)):
-                plan = tune(dA, dB, dC, M, K, N, mode)
-                ms, _ = timed(make_call(dA, dB, dC, M, K, N, mode), max(args.steps, 20), args.warmup)
-                e = err_fp64(dA, dB, dC, K)
+            modes = (("plain", 0), ("split3", 1))
+            plans = {n: tune(dA, dB, dC, M, K, N, m) for n, m in modes}
+            times = rounds({n: make_call(dA, dB, dC, M, K, N, m) for n, m in modes}, max(args.steps, 20), i)
+            for n, m in modes:
+                make_call(dA, dB, dC, M, K, N, m)()
+                e = errors_torch(dA, dB, dC, K)
+                ms = float(np.median(times[n]))
                 flops = 2.0 * M * N * K
                 alg_bytes = 4.0 * (M * K + K * N + M * N)
-                row[mode_name] = {"tflops": flops / (ms * 1e-3) / 1e12, "ms": ms, "plan": list(plan),
-                                  "hbm_gbs": alg_bytes / (ms * 1e-3) / 1e9, "errors": e}
-                log(f"{M}x{N}x{K} {mode_name}: {ms:.4f} ms {row[mode_name]['tflops']:.1f} TFLOP/s "
-                    f"{row[mode_name]['hbm_gbs']:.0f} GB/s plan {plan} ratio {e['bound_ratio']:.3g}")
+                row[n] = {"tflops": flops / (ms * 1e-3) / 1e12, "ms": ms, "spread_pct": spread_pct(times[n]),
+                          "plan": list(plans[n]), "hbm_gbs": alg_bytes / (ms * 1e-3) / 1e9, "errors": e}
+                log(f"{M}x{N}x{K} {n}: {ms:.4f} ms (spread {row[n]['spread_pct']:.1f}%) "
+                    f"{row[n]['tflops']:.1f} TFLOP/s {row[n]['hbm_gbs']:.0f} GB/s plan {plans[n]} "
+                    f"ratio {e['bound_ratio']:.3g}")
             shapes.append(row)
             del dA, dB, dC
         torch.cuda.empty_cache()
 
     grouped = None
     if "grouped" in only:
-        grouped = bench_grouped(torch, mm, inputs, args, log)
+        grouped = bench_grouped(torch, mm, inputs, args, log, rounds)
 
     cpu = None
     if not args.skip_cpu:
@@ -266,17 +295,28 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "data": "synthetic: U[-1,1) FP32 from splitmix64 seeds (A seed 1, B seed 2; narrow shapes 3, 4)",
+        "method": {"rounds": args.rounds, "steps_per_round": args.steps, "order": "rotated per round",
+                   "figure": "median over rounds of ms per call (HIP events around each round's calls; "
+                             "a round runs max(steps, 5 ms of calls))",
+                   "spread": "(max - min) / median over rounds, percent",
+                   "noise_floor_pct": NOISE_FLOOR_PCT,
+                   "source": "bench_final5.cpp:72-151; docs/BENCHMARKS.md:2262-2280"},
     }
     if "main" in out and "plain" in out["main"] and len(out["main"]) > 1:
         head = out["headline_mode"]
         r = out["main"][head]
         line["value"] = r["tflops"]
         line["ms_per_step"] = r["ms"]
+        line["spread_pct"] = r["spread_pct"]
         line["dtype"] = "fp32 in/out; bf16 " + ("3-term" if head == "split3" else "6-term") + " split MFMA, fp32 accumulate"
         line["config"] = {"workload": f"split-BF16 GEMM C=A*B, M=N=K=4096, mode {head}, FP32 inputs/outputs",
                           "M": 4096, "N": 4096, "K": 4096, "mode": head, "plan": r["plan"]}
         line["plain_fp32_mfma_tflops"] = out["main"]["plain"]["tflops"]
         line["speedup_vs_plain"] = r["tflops"] / out["main"]["plain"]["tflops"]
+        line["max_rel_vs_plain"] = r["errors"]["max_rel"] / out["main"]["plain"]["errors"]["max_rel"]
+        line["max_abs_4096"] = {m: out["main"][m]["errors"]["max_abs"] for m in out["main"]}
+        line["reference_max_abs_4096"] = {"value": 2.09e-4, "hardware": "Apple M4, SME 4x1 kernel",
+                                          "source": "docs/BENCHMARKS.md:112-121"}
         line["modes_4096"] = out["main"]
         line["vendor_torch_matmul_4096"] = out["vendor"]
         line["plan_source"] = r["plan_source"]
@@ -366,50 +406,74 @@ def config4_batch(torch, inputs):
     return specs, pool, outs, probs
 
 
-def bench_grouped(torch, mm, inputs, args, log) -> dict:
-    """256 independent GEMMs in one persistent launch replayed from a hipGraph
-    (config4_batch)."""
+def bench_grouped(torch, mm, inputs, args, log, rounds) -> dict:
+    """The 256-problem batch (config4_batch) in one persistent launch replayed
+    from a hipGraph, per mode: the grouped autotuner picks the plans of the
+    heaviest shape classes (the others keep the grouped rule), the launch is
+    timed, and every element of every problem's C is checked against FP64."""
     specs, pool, outs, probs = config4_batch(torch, inputs)
     flops = sum(2.0 * M * K * N for M, K, N in specs)
     alg_bytes = sum(4.0 * (M * K + K * N + M * N) for M, K, N in specs)
     res = {"problems": 256, "total_gflop": flops / 1e9, "total_alg_gb": alg_bytes / 1e9}
-    stream = torch.cuda.current_stream()
-
-    def time_plan(gp, steps):
-        for _ in range(2):
-            gp.launch(stream)
-        torch.cuda.synchronize()
-        t0 = torch.cuda.Event(enable_timing=True)
-        t1 = torch.cuda.Event(enable_timing=True)
-        t0.record(stream)
-        for _ in range(steps):
-            gp.launch(stream)
-        t1.record(stream)
-        torch.cuda.synchronize()
-        return t0.elapsed_time(t1) / steps
-
+    poolv = pool
+    plans_by_mode = {}
+    gps = {}
     for mode_name, mode in (("plain", 0), ("split3", 1)):
-        gp = mm.GroupedPlan(probs, mode=mode)
-        ms = time_plan(gp, 10)
+        rule = mm.grouped_plans(probs, mode)  # fresh process: the grouped cache is empty
+        t = time.perf_counter()
+        tuned, tuned_ms = mm.grouped_autotune(probs, mode, max_classes=args.grouped_classes, reps=3)
+        tune_s = time.perf_counter() - t
+        used = mm.grouped_plans(probs, mode)
+        assert used == tuned, "the grouped cache does not serve the autotuned plans"
+        plans_by_mode[mode_name] = {"autotune_seconds": tune_s, "autotune_launch_ms": tuned_ms,
+                                    "classes_searched": args.grouped_classes,
+                                    "problems_plan_changed_by_autotuner": int(sum(a != b for a, b in zip(rule, used))),
+                                    "distinct_plans": sorted({tuple(p) for p in used})}
+        gps[mode_name] = mm.GroupedPlan(probs, mode=mode)
+        log(f"grouped {mode_name}: autotuned {args.grouped_classes} classes in {tune_s:.1f} s; "
+            f"{plans_by_mode[mode_name]['problems_plan_changed_by_autotuner']} problems changed plan")
+    times = rounds({n: gp.launch for n, gp in gps.items()}, 5)
+    for mode_name, gp in gps.items():
         info = gp.info()
+        # every problem checked against FP64 after a replay
+        for c in outs:
+            c.fill_(float("nan"))
+        gp.launch()
+        torch.cuda.synchronize()
+        worst = {"bound_ratio": 0.0, "max_rel": 0.0, "nonfinite_problems": 0}
+        for (a, b, c_ptr, M, K, N), c in zip(probs, outs):
+            base = poolv.data_ptr()
+            A = poolv[(a - base) // 4:(a - base) // 4 + M * K].view(M, K)
+            B = poolv[(b - base) // 4:(b - base) // 4 + K * N].view(K, N)
+            e = errors_torch(A, B, c.view(M, N), K)
+            if not np.isfinite(e["bound_ratio"]):
+                worst["nonfinite_problems"] += 1
+            worst["bound_ratio"] = max(worst["bound_ratio"], e["bound_ratio"])
+            worst["max_rel"] = max(worst["max_rel"], e["max_rel"])
         gp.close()
-        res[mode_name] = {"ms": ms, "tflops": flops / (ms * 1e-3) / 1e12, "work_items": info["work_items"],
-                          "graph": info["graph"]}
+        ms = float(np.median(times[mode_name]))
+        res[mode_name] = {"ms": ms, "spread_pct": spread_pct(times[mode_name]),
+                          "tflops": flops / (ms * 1e-3) / 1e12, "work_items": info["work_items"],
+                          "graph": info["graph"], "plans": plans_by_mode[mode_name],
+                          "errors_worst_of_256": worst,
+                          "within_fp32_bound": worst["bound_ratio"] <= 1.0 and worst["nonfinite_problems"] == 0}
         log(f"grouped 256 problems {mode_name}: {ms:.3f} ms {res[mode_name]['tflops']:.1f} TFLOP/s "
-            f"items {info['work_items']} graph {info['graph']}")
+            f"items {info['work_items']} worst bound ratio {worst['bound_ratio']:.3g}")
     return res
 
 
 def cpu_baseline(seconds: float) -> dict:
     """The oracle's restatement of the reference's portable FP32 loop
-    (Utils::multiply_scalar), single thread, at 256^3; plus the host BLAS
-    (numpy float32 matmul) at 256^3 and 4096^3 for context."""
+    (Utils::multiply_scalar), single thread, at 256^3, with its error against
+    FP64; plus the host BLAS (numpy float32 matmul) at 256^3 and 4096^3 for
+    context."""
     from oracle import oracle as orc
 
     n = 256
     A = orc.uniform(1, (n, n))
     B = orc.uniform(2, (n, n))
-    orc.sgemm_portable(A, B)  # warm
+    C32 = orc.sgemm_portable(A, B)  # warm, and the result checked below
+    err = orc.errors(C32, orc.gemm_f64(A, B), orc.absgemm_f64(A, B), n)
     reps, t0 = 0, time.perf_counter()
     samples = []
     while time.perf_counter() - t0 < seconds:
@@ -422,7 +486,8 @@ def cpu_baseline(seconds: float) -> dict:
     res = {"value": flops / med / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
            "sample": f"oracle restatement of Utils::multiply_scalar (common/utils.hpp:50-61), "
                      f"M=N=K=256, {reps} calls in {time.perf_counter() - t0:.1f} s, median",
-           "gflops": flops / med / 1e9}
+           "gflops": flops / med / 1e9,
+           "errors_vs_fp64": {k: err[k] for k in ("bound_ratio", "max_rel", "rel_frob", "max_abs")}}
     try:
         info = np.__config__.CONFIG["Build Dependencies"]["blas"]
         blas = f"{info.get('name')} {info.get('version')}"
