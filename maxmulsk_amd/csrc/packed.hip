@@ -259,23 +259,41 @@ struct PackedTile {
                       smem + ((kb + STAGES - 1) % STAGES) * STAGE);
             compute(smem + (kb % STAGES) * STAGE);
         }
-        // Epilogue (same accumulator layout as the direct tile).
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         const int wm = wave / WGN, wn = wave % WGN;
         const int r = lane & 31, h = lane >> 5;
         const long long m0 = (long long)tm * BM, n0 = (long long)tn * BN;
+        // Epilogue through LDS, 32 rows of the wave tile at a time: the
+        // accumulators (lane = row, four columns per register quad) are
+        // written to a padded row-major block, then read back four whole
+        // rows per instruction, so every global store covers complete
+        // 128-byte lines of C instead of 32-byte pieces of 32 rows.
+        constexpr int PITCH = WTN + 4;  // floats; 16-byte row shift keeps both patterns conflict-free
+        static_assert(NW * 32 * PITCH * 4 <= SMEM, "epilogue blocks fit in the ring's LDS");
+        float* blk = reinterpret_cast<float*>(smem) + wave * 32 * PITCH;
+        constexpr int LPR = WTN / 4;  // lanes per row when reading
+        constexpr int RPI = 64 / LPR; // rows per read instruction
+        const int rr = lane / LPR, cc = (lane % LPR) * 4;
+        __syncthreads();  // every wave has finished reading the ring
 #pragma unroll
         for (int ti = 0; ti < TM; ++ti) {
-            const long long m = m0 + wm * WTM + ti * 32 + r;
-            if (m >= M) continue;
-            float* crow = C + m * ldc;
 #pragma unroll
-            for (int tj = 0; tj < TN; ++tj) {
+            for (int tj = 0; tj < TN; ++tj)
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const long long n = n0 + wn * WTN + tj * 32 + 8 * q + 4 * h;
                     const f32x4 v = {acc[ti][tj][4 * q + 0], acc[ti][tj][4 * q + 1], acc[ti][tj][4 * q + 2],
                                      acc[ti][tj][4 * q + 3]};
+                    *reinterpret_cast<f32x4*>(blk + r * PITCH + tj * 32 + 8 * q + 4 * h) = v;
+                }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 32 / RPI; ++i) {
+                const int row = i * RPI + rr;
+                const long long m = m0 + wm * WTM + ti * 32 + row;
+                const long long n = n0 + wn * WTN + cc;
+                const f32x4 v = *reinterpret_cast<const f32x4*>(blk + row * PITCH + cc);
+                if (m < M) {
+                    float* crow = C + m * ldc;
                     if (n + 3 < N) {
                         *reinterpret_cast<f32x4u*>(crow + n) = v;
                     } else {
@@ -285,6 +303,7 @@ struct PackedTile {
                     }
                 }
             }
+            __syncthreads();
         }
     }
 };
