@@ -245,6 +245,15 @@ def main() -> int:
             vendor[name] = {"ms": vms, "tflops": 2.0 * M * N * K / (vms * 1e-3) / 1e12, "spread_pct": spread_pct(vt)}
             log(f"4096^3 torch.matmul {name}: {vms:.4f} ms {vendor[name]['tflops']:.1f} TFLOP/s")
             del a, b, c
+        # What the matrix cores sustain with register operands only (the
+        # library's probe): random operand bits against constant ones.
+        probe = {}
+        for name, mode, iters in (("bf16_32x32x16", 1, 40000), ("fp32_32x32x2", 0, 20000)):
+            probe[name] = {"random_operands_tflops": mm.probe_mfma(mode, True, iters),
+                           "constant_operands_tflops": mm.probe_mfma(mode, False, iters)}
+            log(f"MFMA probe {name}: random {probe[name]['random_operands_tflops']:.0f} TFLOP/s, "
+                f"constant {probe[name]['constant_operands_tflops']:.0f} TFLOP/s")
+        out["mfma_probe"] = probe
         out["main"] = res
         # the headline: the fastest split mode that stays inside the FP32 bound
         ok = [m for m in ("split3", "split6") if m in res and res[m]["errors"]["within_fp32_bound"]]
@@ -332,6 +341,9 @@ def main() -> int:
                             "kernel_ms": kms,
                             "note": f"BF16 MFMA FLOPs = {products} x 2MNK per launch; kernel timed alone on "
                                     "pre-packed operands with HIP events on its stream"}
+        line["roofline"]["frac_of_random_operand_ceiling"] = (
+            achieved / out["mfma_probe"]["bf16_32x32x16"]["random_operands_tflops"])
+        line["mfma_probe"] = out["mfma_probe"]
         line["plain_roofline"] = {"bound": "mfma", "achieved": out["main"]["plain"]["tflops"],
                                   "peak": FP32_PEAK, "unit": "TFLOP/s",
                                   "frac": out["main"]["plain"]["tflops"] / FP32_PEAK}

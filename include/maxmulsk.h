@@ -297,6 +297,15 @@ const char* maxmulsk_phase_counter_name(int i);
 unsigned long long maxmulsk_phase_counter(int i);
 void maxmulsk_phase_counters_reset(void);
 
+/* Matrix-core throughput probe, no counterpart in the reference: every wave
+ * of two 4-wave workgroups per CU issues `iterations` x 8 back-to-back MFMAs
+ * (mode 0: v_mfma_f32_32x32x2_f32, 1: v_mfma_f32_32x32x16_bf16) into eight
+ * accumulators, with register operands: eight different random operand pairs
+ * (random_operands = 1, the bits change on every instruction) or one constant
+ * pair (0). *tflops receives the issued rate on the current device: the
+ * ceiling the GEMM kernels meet on random data, and the cost of the data. */
+maxmulsk_status maxmulsk_probe_mfma(int mode, int random_operands, int iterations, double* tflops);
+
 /* Number of usable gfx950 devices (0 when none). */
 int maxmulsk_device_count(void);
 /* Text of the last error on this thread (never NULL). */

@@ -70,6 +70,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_workspace_bytes": (C.c_size_t, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan)]),
     "maxmulsk_sgemm_ws": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), _P, _SIZE, _P]),
     "maxmulsk_sgemm_noalloc": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), _P]),
+    "maxmulsk_probe_mfma": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "maxmulsk_pack": (C.c_int, [_P, _SIZE, _SIZE, C.c_int, C.c_int, _P, C.POINTER(_P)]),
     "maxmulsk_sgemm_packed": (C.c_int, [_P, _P, _P, C.POINTER(Options)]),
     "maxmulsk_packed_destroy": (None, [_P]),
@@ -220,6 +221,13 @@ def sgemm_ws(A: Any, B: Any, Cm: Any, M: int, K: int, N: int, mode: int, plan: t
     p = C.byref(Plan(*plan)) if plan else None
     return lib().maxmulsk_sgemm_ws(_ptr(A), _ptr(B), _ptr(Cm), M, K, N, mode, p,
                                    _ptr(ws) if ws is not None else None, ws_bytes, _stream_ptr(stream))
+
+
+def probe_mfma(mode: int, random_operands: bool, iterations: int = 20000) -> float:
+    """Issued MFMA rate (TFLOP/s) of the register-only probe (maxmulsk_probe_mfma)."""
+    out = C.c_double()
+    check(lib().maxmulsk_probe_mfma(mode, int(random_operands), iterations, C.byref(out)))
+    return out.value
 
 
 NOALLOC_SCRATCH_BYTES = 32 << 20  # MAXMULSK_NOALLOC_SCRATCH_BYTES
