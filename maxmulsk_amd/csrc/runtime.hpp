@@ -46,6 +46,7 @@ struct DevicePools {
     size_t ws_cap = 0;
     hipEvent_t ws_event = nullptr;
     bool ws_event_recorded = false;
+    hipStream_t ws_stream = nullptr;  // stream of the last recorded use
     void* stage = nullptr;
     size_t stage_cap = 0;
     // Pinned bounce buffers between pageable caller memory and the staging
