@@ -162,6 +162,38 @@ def test_async_calls_alternating_streams_share_the_pool():
         assert full_check(A, B, dC.cpu().numpy())["bound_ratio"] <= 1.0, i
 
 
+def test_async_calls_mixing_the_default_stream_and_a_created_one():
+    """Consecutive calls on the legacy default stream skip the pool's event
+    wait (the stream orders them); a call that switches between it and a
+    created stream must still wait. Each call on the created stream sits
+    behind a bounded GPU sleep, so a missing wait would let the next
+    default-stream call overwrite partials that are still being reduced."""
+    torch = torch_cuda()
+    M, N, K = 128, 1024, 8192
+    side = torch.cuda.Stream()
+    pattern = [0, 0, 1, 0, 0, 1, 1, 0, 1, 0]
+    calls = []
+    for i in range(len(pattern)):
+        A = orc.uniform(950 + 2 * i, (M, K))
+        B = orc.uniform(951 + 2 * i, (K, N))
+        dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+        dC = torch.full((M, N), SENTINEL, dtype=torch.float32, device="cuda")
+        calls.append((A, B, dA, dB, dC))
+    torch.cuda.synchronize()
+    for i, (A, B, dA, dB, dC) in enumerate(calls):
+        stream = 0
+        if pattern[i]:
+            with torch.cuda.stream(side):
+                torch.cuda._sleep(200000)  # bounded spin, about 0.1 ms
+            stream = side.cuda_stream
+        st = mm.sgemm_ex(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, mode=1 + i % 2,
+                         plan=(64, 16), stream=stream, async_=True)
+        assert st == mm.OK, mm.last_error()
+    torch.cuda.synchronize()
+    for i, (A, B, dA, dB, dC) in enumerate(calls):
+        assert full_check(A, B, dC.cpu().numpy())["bound_ratio"] <= 1.0, i
+
+
 def test_synchronous_calls_on_different_streams_overlap():
     """A synchronous call waits for its own stream only: thread A's call sits
     behind a bounded GPU sleep on stream A while thread B's call on stream B
