@@ -165,7 +165,7 @@ extern "C" int maxmulsk_tile_info(int cfg, int mode, int* out) {
     const int waves = cfg < 5 ? direct_wgm(mode, cfg) * direct_wgn(mode, cfg) : t.wgm * t.wgn;
     out[0] = t.bm;
     out[1] = t.bn;
-    out[2] = t.bk[mode];
+    out[2] = cfg == kPackedCfg ? packed_bk(mode) : t.bk[mode];
     out[3] = 64 * waves;
     out[4] = b_unit_cols(mode, t.bn, t.bk[mode], waves);  // B staging unit width of the tile kernel
     out[5] = waves;

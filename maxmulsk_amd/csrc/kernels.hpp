@@ -10,7 +10,7 @@ namespace mmsk {
 // Tile geometries: wgm x wgn waves over the output tile. Geometries 0..4 are
 // compiled for every mode as the direct tile kernel (and serve the grouped
 // kernel); geometry 5 (256 x 256, eight waves, two per SIMD) runs on the
-// packed path, whose k-block depth is kPackedBK.
+// packed path, whose k-block depth is packed_bk(mode).
 struct TileCfg {
     int bm, bn;
     int wgm, wgn;
@@ -87,8 +87,15 @@ struct GemmParams {
 hipError_t launch_gemm(int mode, int cfg, const GemmParams& p, hipStream_t stream);
 
 // ---- packed path (256 x 256 geometry) -------------------------------------------
-// k-block depth of the packed GEMM and k values per packed fragment.
-constexpr int kPackedBK = 16;
+// k-block depth of the packed GEMM per mode and k values per packed fragment.
+// Plain and split3 take 32-deep k-blocks in a two-stage ring (128 KiB of LDS):
+// half the barriers of 16-deep blocks in a four-stage ring, measured 1.5-2%
+// faster at 4096^3 and 4096 x 24576 x 1536 with bit-identical results; split6
+// keeps 16 (a 32-deep stage of its three terms is 96 KiB).
+constexpr int packed_bk(int mode) { return mode == 2 ? 16 : 32; }
+static_assert(kTileCfgs[5].bk[0] == packed_bk(0) && kTileCfgs[5].bk[1] == packed_bk(1) &&
+                  kTileCfgs[5].bk[2] == packed_bk(2),
+              "the tile table lists the packed k-block depths");
 constexpr int kPackedCfg = 5;
 constexpr int packed_kgk(int mode) { return mode == 0 ? 8 : 16; }
 
@@ -100,7 +107,7 @@ struct PackedParams {
     float* C;
     long long M, N, ldc;
     long long c_split_stride;
-    long long kblocks;            // k-blocks of kPackedBK in the packed operands
+    long long kblocks;            // k-blocks of packed_bk(mode) in the packed operands
     long long kblocks_per_split;
     int splits, tiles_m, tiles_n;
 };

@@ -311,11 +311,14 @@ struct PackedTile {
 template <int MODE>
 struct PackedCfg;
 // 256 x 256 tiles, eight waves of 128 x 64 (two waves per SIMD, so one
-// wave's LDS reads and DMA issue overlap the other's MFMAs); k-block depth and
-// ring depth per mode.
-template <> struct PackedCfg<0> { using T = PackedTile<0, 256, 256, 16, 2, 4, 4>; };
-template <> struct PackedCfg<1> { using T = PackedTile<1, 256, 256, 16, 2, 4, 4>; };
+// wave's LDS reads and DMA issue overlap the other's MFMAs); k-block depth
+// (packed_bk) and ring depth per mode.
+template <> struct PackedCfg<0> { using T = PackedTile<0, 256, 256, 32, 2, 4, 2>; };
+template <> struct PackedCfg<1> { using T = PackedTile<1, 256, 256, 32, 2, 4, 2>; };
 template <> struct PackedCfg<2> { using T = PackedTile<2, 256, 256, 16, 2, 4, 3>; };
+static_assert(PackedCfg<0>::T::KGB * PackedCfg<0>::T::KGK == packed_bk(0), "plain k-block depth");
+static_assert(PackedCfg<1>::T::KGB * PackedCfg<1>::T::KGK == packed_bk(1), "split3 k-block depth");
+static_assert(PackedCfg<2>::T::KGB * PackedCfg<2>::T::KGK == packed_bk(2), "split6 k-block depth");
 
 template <int MODE>
 __global__ __launch_bounds__(PackedCfg<MODE>::T::NT) void gemm_packed_kernel(PackedParams p) {

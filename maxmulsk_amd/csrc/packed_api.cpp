@@ -21,7 +21,7 @@ struct maxmulsk_packed_s {
     int operand;      // 0: A (rows = M), 1: B (rows = N)
     size_t rows, K;
     int rblocks;      // 32-row fragment blocks, a multiple of 8 (one 256-row tile)
-    int kgroups;      // fragment k-groups, kblocks * (kPackedBK / packed_kgk(mode))
+    int kgroups;      // fragment k-groups, kblocks * (packed_bk(mode) / packed_kgk(mode))
     long long kblocks;
     long long plane;  // bytes of one term plane
     void* buf;
@@ -49,8 +49,8 @@ extern "C" maxmulsk_status maxmulsk_pack(const float* X, size_t rows, size_t K, 
     if (!is_device_ptr(X, &dev)) return fail(MAXMULSK_ERR_UNSUPPORTED, "packing needs a device pointer");
     if (!device_usable(dev)) return fail(MAXMULSK_ERR_UNSUPPORTED, "device %d is not gfx950", dev);
 
-    const long long kblocks = ((long long)K + kPackedBK - 1) / kPackedBK;
-    const int kgroups = (int)(kblocks * (kPackedBK / packed_kgk(md)));
+    const long long kblocks = ((long long)K + packed_bk(md) - 1) / packed_bk(md);
+    const int kgroups = (int)(kblocks * (packed_bk(md) / packed_kgk(md)));
     const int rblocks = (int)(((long long)rows + kTile - 1) / kTile * (kTile / 32));
     const long long plane = 1024LL * kgroups * rblocks;
     const size_t bytes = (size_t)plane * terms_of(md);

@@ -50,7 +50,7 @@ bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* 
     r.bm = tc.bm;
     r.bn = tc.bn;
     r.packed = (cfg == kPackedCfg);
-    r.bk = r.packed ? kPackedBK : tc.bk[mode];
+    r.bk = r.packed ? packed_bk(mode) : tc.bk[mode];
     const long long kblocks = ((long long)K + r.bk - 1) / r.bk;
     long long per = (kblocks + p.split_k - 1) / p.split_k;  // k-blocks per slice
     if (per < 1) per = 1;
@@ -67,7 +67,7 @@ bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* 
     r.pack_a_bytes = r.pack_b_bytes = 0;
     if (r.packed) {
         const int terms = mode == 0 ? 1 : (mode == 1 ? 2 : 3);
-        r.kgroups = (int)(kblocks * (kPackedBK / packed_kgk(mode)));
+        r.kgroups = (int)(kblocks * (packed_bk(mode) / packed_kgk(mode)));
         const unsigned long long frag = 1024ULL * (unsigned long long)r.kgroups * terms;
         r.pack_a_bytes = up(frag * (unsigned long long)r.tiles_m * (r.bm / 32));
         r.pack_b_bytes = up(frag * (unsigned long long)r.tiles_n * (r.bn / 32));

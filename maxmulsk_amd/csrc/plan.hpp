@@ -36,7 +36,7 @@ struct Resolved {
     // Packed path (256 x 256 geometry): operands rewritten once into MFMA
     // fragments in the workspace, after the partials.
     bool packed;
-    long long kblocks;   // packed k-blocks (kPackedBK deep)
+    long long kblocks;   // packed k-blocks (packed_bk(mode) deep)
     int kgroups;         // fragments per row block along K
     size_t partial_bytes, pack_a_bytes, pack_b_bytes;
 };

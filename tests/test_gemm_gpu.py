@@ -36,10 +36,12 @@ def _up256(x):
 def test_every_split_count_reduces_exactly(mode):
     """Plan (tile, 32) on a K of s k-blocks runs exactly s slices, so the reduce
     sees every split count 1..32 (direct 64-row tiles: k-blocks of 32; packed
-    256 x 256 tiles: k-blocks of 16). Small integers keep every partial sum
-    exact, so the result must equal the integer product bit for bit."""
+    256 x 256 tiles: k-blocks of 32, or 16 in split6). Small integers keep every
+    partial sum exact, so the result must equal the integer product bit for bit."""
+    packed_depth = mm.tile_info(5, mode)["bk"]
+    assert packed_depth == (16 if mode == 2 else 32)
     for s in range(1, 33):
-        for M, N, tile, depth in ((67, 93, 64, 32), (300, 260, 256, 16)):
+        for M, N, tile, depth in ((67, 93, 64, 32), (300, 260, 256, packed_depth)):
             K = depth * s
             extra = mm.workspace_bytes(M, K, N, mode, (tile, 32)) - (
                 mm.workspace_bytes(M, K, N, mode, (tile, 1)))
