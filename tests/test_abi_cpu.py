@@ -159,11 +159,15 @@ def test_workspace_bytes():
     assert mm.workspace_bytes(64, 8192, 512, 0, (64, 16)) == 16 * 64 * 512 * 4
     # 256-wide tiles on more than 128 rows take the packed path: the workspace
     # holds A and B rewritten as fragments (FP32 copy, or 2 / 3 BF16 terms),
-    # padded to whole 256-row tiles and 16-deep k-blocks
+    # padded to whole 256-row tiles and whole packed k-blocks (32 deep in plain
+    # and split3, 16 in split6)
     assert mm.workspace_bytes(4096, 4096, 4096, 0, (256, 1)) == 2 * 4096 * 4096 * 4
     assert mm.workspace_bytes(4096, 4096, 4096, 1, (256, 1)) == 2 * 2 * 4096 * 4096 * 2
     assert mm.workspace_bytes(4096, 4096, 4096, 2, (256, 1)) == 2 * 3 * 4096 * 4096 * 2
     assert mm.workspace_bytes(300, 20, 300, 1, (256, 1)) == 2 * (2 * 512 * 32 * 2)
+    assert mm.workspace_bytes(300, 40, 300, 0, (256, 1)) == 2 * (512 * 64 * 4)
+    assert mm.workspace_bytes(300, 40, 300, 1, (256, 1)) == 2 * (2 * 512 * 64 * 2)
+    assert mm.workspace_bytes(300, 40, 300, 2, (256, 1)) == 2 * (3 * 512 * 48 * 2)
     assert mm.workspace_bytes(4096, 4096, 4096, 0, (128, 1)) == 0
     # split clamps to the number of k-blocks: K=64 with BK=32 has 2 blocks
     assert mm.workspace_bytes(64, 64, 64, 0, (64, 16)) == 2 * 64 * 64 * 4
