@@ -260,12 +260,13 @@ maxmulsk_status pool_workspace_acquire(DevicePools& pl, size_t bytes, hipStream_
     // A use on the stream of the previous use is ordered behind it by the
     // stream itself, so the wait is only needed across streams. It is not
     // free: measured on a 64 x 512 x 8192 split-K call, the wait cost 6.5 us
-    // of host time and 3 us of stream time per call. It is skipped only for
-    // the two default streams, which are never destroyed: a created stream's
-    // handle can be reused by a new stream after hipStreamDestroy, which may
-    // return while the old stream's work is still running.
-    const bool default_stream = stream == nullptr || stream == hipStreamPerThread;
-    const bool same = default_stream && stream == pl.ws_stream;
+    // of host time and 3 us of stream time per call. It is skipped only on
+    // the device's legacy default stream, which is one stream for every thread
+    // and is never destroyed. A created stream's handle can be reused by a new
+    // stream after hipStreamDestroy, which may return while the old stream's
+    // work is still running; hipStreamPerThread names a different stream in
+    // each thread.
+    const bool same = stream == nullptr && pl.ws_stream == nullptr;
     if (pl.ws_event_recorded && !same && hipStreamWaitEvent(stream, pl.ws_event, 0) != hipSuccess) {
         (void)hipGetLastError();
         return fail(MAXMULSK_ERR_DEVICE, "stream wait failed");
