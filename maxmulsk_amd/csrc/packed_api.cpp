@@ -147,7 +147,12 @@ extern "C" maxmulsk_status maxmulsk_sgemm_packed(maxmulsk_packed A, maxmulsk_pac
         e = launch_splitk_reduce(partials, C, (long long)(M * N), splits, (long long)(M * N), s);
         count(kCntReduce);
     }
-    if (e == hipSuccess && pooled) e = pool_workspace_release(pl, s);
+    if (pooled) {
+        // Also after a failed launch: whatever was enqueued before it must be
+        // covered for the next user of the workspace.
+        const hipError_t er = pool_workspace_release(pl, s);
+        if (e == hipSuccess) e = er;
+    }
     lock.unlock();  // the wait below needs nothing from the pools
     if (e == hipSuccess && !async) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {

@@ -258,16 +258,26 @@ maxmulsk_status pool_workspace_acquire(DevicePools& pl, size_t bytes, hipStream_
         return fail(MAXMULSK_ERR_DEVICE, "event creation failed");
     }
     // A use on the stream of the previous use is ordered behind it by the
-    // stream itself, so the wait is only needed across streams. It is not
-    // free: measured on a 64 x 512 x 8192 split-K call, the wait cost 6.5 us
-    // of host time and 3 us of stream time per call. It is skipped only on
-    // the device's legacy default stream, which is one stream for every thread
-    // and is never destroyed. A created stream's handle can be reused by a new
-    // stream after hipStreamDestroy, which may return while the old stream's
-    // work is still running; hipStreamPerThread names a different stream in
-    // each thread.
-    const bool same = stream == nullptr && pl.ws_stream == nullptr;
-    if (pl.ws_event_recorded && !same && hipStreamWaitEvent(stream, pl.ws_event, 0) != hipSuccess) {
+    // stream itself, so the event is only needed across streams, and it is
+    // not free: on a 64 x 512 x 8192 split-K call its record cost about
+    // 1.6 us of stream time, and the wait 6.5 us of host time and 3 us of
+    // stream time. Uses on the device's legacy default stream therefore skip
+    // both: that stream is one stream for every thread and is never
+    // destroyed, so when a use on another stream follows, the event is
+    // recorded on the default stream then, behind everything enqueued there
+    // so far. Uses on created streams record at release and are always waited
+    // for: hipStreamDestroy may return while a stream's work still runs, after
+    // which a new stream can reuse the handle, and hipStreamPerThread names a
+    // different stream in each thread.
+    if (pl.ws_used && pl.ws_stream == nullptr) {
+        if (stream != nullptr) {
+            if (hipEventRecord(pl.ws_event, nullptr) != hipSuccess ||
+                hipStreamWaitEvent(stream, pl.ws_event, 0) != hipSuccess) {
+                (void)hipGetLastError();
+                return fail(MAXMULSK_ERR_DEVICE, "stream wait failed");
+            }
+        }
+    } else if (pl.ws_event_recorded && hipStreamWaitEvent(stream, pl.ws_event, 0) != hipSuccess) {
         (void)hipGetLastError();
         return fail(MAXMULSK_ERR_DEVICE, "stream wait failed");
     }
@@ -276,9 +286,14 @@ maxmulsk_status pool_workspace_acquire(DevicePools& pl, size_t bytes, hipStream_
 }
 
 hipError_t pool_workspace_release(DevicePools& pl, hipStream_t stream) {
+    pl.ws_used = true;
+    pl.ws_stream = stream;
+    if (stream == nullptr) {
+        pl.ws_event_recorded = false;  // recorded by the next use on another stream
+        return hipSuccess;
+    }
     const hipError_t e = hipEventRecord(pl.ws_event, stream);
     pl.ws_event_recorded = (e == hipSuccess);
-    pl.ws_stream = stream;
     return e;
 }
 
@@ -484,7 +499,12 @@ maxmulsk_status run_gemm(const GemmCall& c) {
         Scope scope("maxmulsk/launch");
         e = launch_resolved(A, B, C, c.M, c.K, c.N, c.mode, r, ws, c.stream);
     }
-    if (e == hipSuccess && pooled) e = pool_workspace_release(pl, c.stream);
+    if (pooled) {
+        // Also after a failed launch: whatever was enqueued before it must be
+        // covered for the next user of the workspace.
+        const hipError_t er = pool_workspace_release(pl, c.stream);
+        if (e == hipSuccess) e = er;
+    }
     if (e == hipSuccess && staged && !c_dev) {
         Scope scope("maxmulsk/writeback");
         e = stage_out(pl, c.C, C, c.M * c.N * sizeof(float), c.stream);

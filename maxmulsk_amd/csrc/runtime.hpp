@@ -45,8 +45,9 @@ struct DevicePools {
     void* ws = nullptr;
     size_t ws_cap = 0;
     hipEvent_t ws_event = nullptr;
-    bool ws_event_recorded = false;
-    hipStream_t ws_stream = nullptr;  // stream of the last recorded use
+    bool ws_event_recorded = false;   // ws_event covers the last use
+    bool ws_used = false;             // the workspace has been handed out before
+    hipStream_t ws_stream = nullptr;  // stream of the last use
     void* stage = nullptr;
     size_t stage_cap = 0;
     // Pinned bounce buffers between pageable caller memory and the staging
