@@ -76,7 +76,11 @@ typedef struct {
 } maxmulsk_options;
 
 /* C = A*B with an explicit mode, plan, stream and workspace. opt may be NULL
- * (plain mode, planner, default stream, synchronous). */
+ * (plain mode, planner, default stream, synchronous). Without a workspace, a
+ * plan that needs one (split-K partials, packed operands) takes the library's
+ * per-device pooled workspace: a call on another stream than the previous user
+ * waits for it on the device (an event), while consecutive calls on the legacy
+ * default stream are ordered by that stream alone. */
 maxmulsk_status maxmulsk_sgemm_ex(const float* A, const float* B, float* C,
                                   size_t M, size_t K, size_t N,
                                   const maxmulsk_options* opt);
