@@ -53,7 +53,9 @@ maxmulsk_status maxmulsk_sgemm_v4(const float* A, const float* B, float* C,
 
 typedef enum {
     MAXMULSK_MODE_PLAIN  = 0,  /* FP32 operands, v_mfma_f32_32x32x2_f32, FP32 accumulate */
-    MAXMULSK_MODE_SPLIT3 = 1,  /* a = hi + lo (BF16), hi*hi + hi*lo + lo*hi on BF16 MFMA */
+    MAXMULSK_MODE_SPLIT3 = 1,  /* a = hi + lo (BF16), hi*hi + hi*lo + lo*hi on BF16 MFMA; error
+                                * within (K * 2^-24 + 3 * 2^-16) * |A||B|: the FP32 bound for long
+                                * K, not for short K (about K < 64 on uniform data) */
     MAXMULSK_MODE_SPLIT6 = 2,  /* a = hi + mid + lo (BF16), six cross products on BF16 MFMA */
 } maxmulsk_mode;
 

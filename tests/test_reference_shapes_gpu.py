@@ -85,3 +85,39 @@ def test_reference_shapes_repeat_call_with_new_b():
             e = go.check(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N)
             assert e["nonfinite"] == 0 and e["bound_ratio"] <= 1.0, (M, K, N, rep, e)
         del dA, dB, dC
+
+
+# The reference's GEMM-vs-scalar unit-test shapes (test_sme.cpp:28-37,431-453), as (M, K, N).
+UNIT_SHAPES = [(16, 16, 16), (32, 32, 32), (64, 64, 64), (67, 67, 67), (128, 128, 128), (20, 35, 41),
+               (20, 41, 35), (256, 256, 256), (512, 512, 512)]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_reference_unit_shapes_every_plan(mode):
+    """Every candidate plan on the reference's unit-test shapes, every element of
+    C within its mode's bound (the reference held these to 1e-3 absolute against
+    its scalar loop, utils.hpp:24-25). Plain and split6: the FP32 bound
+    K * 2^-24 * |A||B|. Split3 drops lo*lo and each operand's representation
+    error, up to 3 * 2^-16 * |a||b| per product whatever K is, so its bound is
+    (K * 2^-24 + 3 * 2^-16) * |A||B|: a bound ratio of 1 + 768 / K against the
+    FP32 bound, which it exceeds on uniform data below K of about 64."""
+    torch = torch_cuda()
+    from oracle import gpu_oracle as go
+
+    gen = torch.Generator(device="cuda")
+    for M, K, N in UNIT_SHAPES:
+        gen.manual_seed(M * 7 + K * 5 + N)
+        dA, dB = _uniform(torch, gen, M * K), _uniform(torch, gen, K * N)
+        dC = torch.empty((M * N,), dtype=torch.float32, device="cuda")
+        for plan in mm.candidates():
+            dC.fill_(SENTINEL)
+            torch.cuda.synchronize()
+            st = mm.sgemm_ex(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N, mode=mode, plan=plan)
+            assert st == mm.OK, (M, K, N, plan, mm.last_error())
+            torch.cuda.synchronize()
+            e = go.check(dA.data_ptr(), dB.data_ptr(), dC.data_ptr(), M, K, N)
+            limit = 1.0 + 768.0 / K if mode == 1 else 1.0
+            assert e["nonfinite"] == 0 and e["bound_ratio"] <= limit, (mode, M, K, N, plan, e)
+            assert e["max_abs"] <= 1e-3, (mode, M, K, N, plan, e)
+            if plan == (64, 1):
+                report("unit_shapes", shape_mkn=[M, K, N], mode=mode, **e)
