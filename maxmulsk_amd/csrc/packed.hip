@@ -178,8 +178,9 @@ struct PackedTile {
         }
     }
 
-    // B fragments once per k-group, A fragments one 32-row block at a time
-    // (the other wave on the SIMD covers the LDS latency).
+    // Per k-group: plain mode reads B fragments once and A fragments one 32-row
+    // block at a time (the other wave on the SIMD covers the LDS latency); the
+    // split modes read every fragment of the k-group first.
     __device__ __forceinline__ void compute(const char* st) {
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         const int wm = wave / WGN, wn = wave % WGN;
@@ -209,24 +210,29 @@ struct PackedTile {
                     for (int tj = 0; tj < TN; ++tj)
                         b[s][tj] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
                                                                   sb + ((s * RB + wn * TN + tj) * KGB + kg) * 1024));
+                // All A fragments of the k-group, then for each B column block the products in
+                // order (smallest first per accumulator) over the TM row blocks: consecutive
+                // MFMAs share their B operand, which measured 1.5-2% faster at 4096^3
+                // (split3 and split6) than sharing the A operand two at a time.
+                bf16x8 a[TERMS][TM];
 #pragma unroll
-                for (int ti = 0; ti < TM; ++ti) {
-                    bf16x8 a[TERMS];
+                for (int ti = 0; ti < TM; ++ti)
 #pragma unroll
                     for (int s = 0; s < TERMS; ++s)
-                        a[s] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
+                        a[s][ti] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(
                                                               sa + ((s * RA + wm * TM + ti) * KGB + kg) * 1024));
-                    constexpr int NP = MODE == 1 ? 3 : 6;
+                constexpr int NP = MODE == 1 ? 3 : 6;
+#pragma unroll
+                for (int tj = 0; tj < TN; ++tj)
 #pragma unroll
                     for (int p = 0; p < NP; ++p) {
                         const int ta = product_term(MODE, p, 0);
                         const int tb = product_term(MODE, p, 1);
 #pragma unroll
-                        for (int tj = 0; tj < TN; ++tj)
+                        for (int ti = 0; ti < TM; ++ti)
                             acc[ti][tj] =
-                                __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[tb][tj], a[ta], acc[ti][tj], 0, 0, 0);
+                                __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[tb][tj], a[ta][ti], acc[ti][tj], 0, 0, 0);
                     }
-                }
             }
         }
     }
