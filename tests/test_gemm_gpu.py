@@ -215,6 +215,33 @@ def test_v3_v4_and_host_pointers_agree():
         assert full_check(A, B, Cx)["bound_ratio"] <= 1.0
 
 
+@pytest.mark.parametrize("name,mode", [("split3", 1), ("split6", 2)])
+def test_mode_environment_switches_the_reference_entry_points(monkeypatch, name, mode):
+    """MAXMULSK_MODE switches maxmulsk_sgemm_v4 and v3 to a split mode, read on
+    every call: v4 then equals maxmulsk_sgemm_ex in that mode with the same
+    planner bit for bit, v3 equals sgemm_ex with the shape rule's plan, and
+    both differ from plain on data that is not exact in BF16."""
+    M, N, K = 256, 384, 2048
+    A = orc.uniform(61, (M, K))
+    B = orc.uniform(62, (K, N))
+    torch_cuda()
+    plain = gpu_gemm(A, B, entry="v4")
+    monkeypatch.setenv("MAXMULSK_MODE", name)
+    C4 = gpu_gemm(A, B, entry="v4")
+    C3 = gpu_gemm(A, B, entry="v3")
+    monkeypatch.delenv("MAXMULSK_MODE")
+    Cex = gpu_gemm(A, B, mode=mode)
+    Crule = gpu_gemm(A, B, mode=mode, plan=mm.rule_plan(M, K, N, mode))
+    assert np.array_equal(C4, Cex)
+    assert np.array_equal(C3, Crule)
+    assert not np.array_equal(C4, plain)
+    for Cx in (C3, C4):
+        assert full_check(A, B, Cx)["bound_ratio"] <= 1.0
+    # an unknown name leaves plain mode
+    monkeypatch.setenv("MAXMULSK_MODE", "split4")
+    assert np.array_equal(gpu_gemm(A, B, entry="v4"), plain)
+
+
 @pytest.mark.parametrize("plan", [(128, 1), (256, 2)])
 def test_host_operands_larger_than_the_staging_chunks(plan):
     """Host operands that span several 8 MiB staging chunks, with ragged sizes,
