@@ -312,6 +312,32 @@ void maxmulsk_phase_counters_reset(void);
  * ceiling the GEMM kernels meet on random data, and the cost of the data. */
 maxmulsk_status maxmulsk_probe_mfma(int mode, int random_operands, int iterations, double* tflops);
 
+/* Operand feed of the 64 x 64 tile (plans with tile 64, and narrow-M plans
+ * whose tile shrinks to 64 x 64). MAXMULSK_FEED_DIRECT: each wave stages,
+ * splits and multiplies (the direct tile). MAXMULSK_FEED_DECOUPLED: every such
+ * launch runs four producer waves that stage and split k-blocks into an LDS
+ * ring and four consumer waves that only multiply and write C, handing slots
+ * over through ready/consumed counters. MAXMULSK_FEED_AUTO (the default): the
+ * decoupled feed where it measured faster, i.e. launches of at most one
+ * workgroup per CU with K slices of at least 512 (split6) or of 64 to 256
+ * workgroups with slices of at least 1024 (split3); the direct tile elsewhere. Both feeds give the same C (same K slices, products and
+ * order), except that an exact -0 may come out as +0. Counterpart of the
+ * reference's Trio engine, one compute thread fed by separate packing threads
+ * through rings (sme/trio/trio.hpp:10-24, sme/trio/trio_sme.cpp:274-334). The
+ * setting is process-wide; the first use reads MAXMULSK_FEED=direct|
+ * decoupled (anything else: auto). maxmulsk_set_feed returns the previous
+ * setting, or -1 for an unknown value (nothing changes). */
+#define MAXMULSK_FEED_DIRECT 0
+#define MAXMULSK_FEED_DECOUPLED 1
+#define MAXMULSK_FEED_AUTO 2
+int maxmulsk_set_feed(int feed);
+int maxmulsk_get_feed(void);
+/* Kernel a call with this shape, mode and plan launches under the current
+ * feed setting: 0 = the direct tile (or a geometry other than 64 x 64),
+ * 1..6 = a decoupled-feed variant (k-block depth, ring slots and loads in
+ * flight differ per variant), -1 = invalid plan. */
+int maxmulsk_feed_variant(size_t M, size_t K, size_t N, maxmulsk_mode mode, const maxmulsk_plan* plan);
+
 /* Number of usable gfx950 devices (0 when none). */
 int maxmulsk_device_count(void);
 /* Text of the last error on this thread (never NULL). */

@@ -71,6 +71,9 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_sgemm_ws": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), _P, _SIZE, _P]),
     "maxmulsk_sgemm_noalloc": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), _P]),
     "maxmulsk_probe_mfma": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "maxmulsk_set_feed": (C.c_int, [C.c_int]),
+    "maxmulsk_get_feed": (C.c_int, []),
+    "maxmulsk_feed_variant": (C.c_int, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan)]),
     "maxmulsk_pack": (C.c_int, [_P, _SIZE, _SIZE, C.c_int, C.c_int, _P, C.POINTER(_P)]),
     "maxmulsk_sgemm_packed": (C.c_int, [_P, _P, _P, C.POINTER(Options)]),
     "maxmulsk_packed_destroy": (None, [_P]),
@@ -228,6 +231,28 @@ def probe_mfma(mode: int, random_operands: bool, iterations: int = 20000) -> flo
     out = C.c_double()
     check(lib().maxmulsk_probe_mfma(mode, int(random_operands), iterations, C.byref(out)))
     return out.value
+
+
+FEED_DIRECT = 0     # MAXMULSK_FEED_DIRECT
+FEED_DECOUPLED = 1  # MAXMULSK_FEED_DECOUPLED
+FEED_AUTO = 2       # MAXMULSK_FEED_AUTO
+
+
+def set_feed(feed: int) -> int:
+    """maxmulsk_set_feed: operand feed of the 64 x 64 tile; returns the previous one."""
+    prev = lib().maxmulsk_set_feed(int(feed))
+    if prev < 0:
+        raise ValueError(f"unknown feed {feed}")
+    return prev
+
+
+def get_feed() -> int:
+    return lib().maxmulsk_get_feed()
+
+
+def feed_variant(M: int, K: int, N: int, mode: int, plan: tuple[int, int]) -> int:
+    """maxmulsk_feed_variant: 0 = direct tile, 1..6 = decoupled-feed variant, -1 = invalid plan."""
+    return lib().maxmulsk_feed_variant(M, K, N, mode, C.byref(Plan(*plan)))
 
 
 NOALLOC_SCRATCH_BYTES = 32 << 20  # MAXMULSK_NOALLOC_SCRATCH_BYTES

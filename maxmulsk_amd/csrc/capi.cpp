@@ -179,3 +179,20 @@ extern "C" const char* maxmulsk_phase_counter_name(int i) { return counter_name(
 extern "C" unsigned long long maxmulsk_phase_counter(int i) { return (unsigned long long)counter(i); }
 
 extern "C" void maxmulsk_phase_counters_reset(void) { reset_counters(); }
+
+extern "C" int maxmulsk_set_feed(int feed) {
+    if (feed < MAXMULSK_FEED_DIRECT || feed > MAXMULSK_FEED_AUTO) return -1;
+    const int prev = feed_mode();
+    set_feed_mode(feed);
+    return prev;
+}
+
+extern "C" int maxmulsk_get_feed(void) { return feed_mode(); }
+
+extern "C" int maxmulsk_feed_variant(size_t M, size_t K, size_t N, maxmulsk_mode mode, const maxmulsk_plan* plan) {
+    if (!plan) return -1;
+    Resolved r{};
+    if (!resolve(M, K, N, (int)mode, *plan, &r)) return -1;
+    if (r.cfg != 0) return 0;
+    return feed_variant((int)mode, (long long)r.tiles_m * r.tiles_n * r.splits, r.k_len, feed_mode());
+}

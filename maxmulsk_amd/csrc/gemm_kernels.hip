@@ -68,6 +68,8 @@ template <int MODE>
 static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) {
     switch (cfg) {
         case 0:
+            if (const int fv = feed_variant(MODE, (long long)p.tiles_m * p.tiles_n * p.splits, p.k_len, feed_mode()))
+                return launch_gemm_feed(fv, p, stream);
             if constexpr (kDeepBK[MODE] > 0) {
                 const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
                 if (grid > 0 && grid <= kDeepMaxGrid) return launch_deep<MODE, kDeepBK[MODE]>(p, stream);

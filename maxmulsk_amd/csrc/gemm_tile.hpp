@@ -127,7 +127,7 @@ struct Tile {
 
     // Register sets for k-blocks in flight between global memory and LDS.
     static constexpr int PF = PF_;
-    static_assert(PF == 1 || PF == 2, "one or two register sets");
+    static_assert(PF >= 1 && PF <= 4, "one to four register sets");
     f32x4 ra[PF][UA][LU / 4];  // A unit i: LU consecutive k of one row
     using bvec = float __attribute__((ext_vector_type(BW)));
     using bvecu = float __attribute__((ext_vector_type(BW), aligned(4)));
@@ -143,7 +143,16 @@ struct Tile {
                                          long long M, long long N, long long lda, long long ldb,
                                          long long m0, long long n0, long long k0, long long kend,
                                          bool fast) {
-        const int tid = threadIdx.x;
+        load_as<S>(threadIdx.x, A, B, M, N, lda, ldb, m0, n0, k0, kend, fast);
+    }
+
+    // load() for staging thread `tid` (0..NT-1) of the tile, which need not be
+    // threadIdx.x (the decoupled feed stages with waves of its own).
+    template <int S>
+    __device__ __forceinline__ void load_as(int tid, const float* __restrict__ A, const float* __restrict__ B,
+                                            long long M, long long N, long long lda, long long ldb,
+                                            long long m0, long long n0, long long k0, long long kend,
+                                            bool fast) {
 #pragma unroll
         for (int i = 0; i < UA; ++i) {
             const int u = tid + NT * i;
@@ -190,8 +199,10 @@ struct Tile {
     }
 
     template <int S>
-    __device__ __forceinline__ void store(char* stage) const {
-        const int tid = threadIdx.x;
+    __device__ __forceinline__ void store(char* stage) const { store_as<S>(threadIdx.x, stage); }
+
+    template <int S>
+    __device__ __forceinline__ void store_as(int tid, char* stage) const {
 #pragma unroll
         for (int i = 0; i < UA; ++i) {
             const int u = tid + NT * i;
@@ -362,6 +373,7 @@ struct Tile {
                                              long long M, long long N, long long lda, long long ldb,
                                              long long m0, long long n0, long long kbeg, long long kend,
                                              bool full_mn, char* smem) {
+        static_assert(PF <= 2, "the tile's own loop runs one or two register sets");
         const long long nkb = (kend - kbeg + BK - 1) / BK;
         auto fast = [&](long long k0) { return ALL_FAST || (full_mn && k0 + BK <= kend); };
         if constexpr (PF == 1) {

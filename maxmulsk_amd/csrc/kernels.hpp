@@ -86,6 +86,19 @@ struct GemmParams {
 
 hipError_t launch_gemm(int mode, int cfg, const GemmParams& p, hipStream_t stream);
 
+// Decoupled producer/consumer feed of geometry 0 (feed.hip): four waves stage
+// and split operands into an LDS ring, four others only multiply; same K
+// slices, products and order as the direct 64 x 64 tile. feed_mode(): 0 =
+// direct tile kernels only, 1 = every geometry-0 launch takes the feed, 2 =
+// the feed where it measured faster (the default). MAXMULSK_FEED=direct|
+// decoupled sets 0 or 1 at first use; set_feed_mode overrides.
+int feed_mode();
+void set_feed_mode(int v);
+// Feed variant of a geometry-0 launch of `grid` workgroups and K slices of
+// k_len under a feed setting; 0 = the direct tile.
+int feed_variant(int mode, long long grid, long long k_len, int setting);
+hipError_t launch_gemm_feed(int variant, const GemmParams& p, hipStream_t stream);
+
 // ---- packed path (256 x 256 geometry) -------------------------------------------
 // k-block depth of the packed GEMM per mode and k values per packed fragment.
 // Plain and split3 take 32-deep k-blocks in a two-stage ring (128 KiB of LDS):

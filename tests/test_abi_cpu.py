@@ -384,3 +384,49 @@ def test_product_input_generator_matches_oracle():
     from oracle import oracle as orc
     assert np.array_equal(inputs.uniform(9, (123, 45)), orc.uniform(9, (123, 45)))
     assert np.array_equal(inputs.log_uniform(9, (77,)), orc.log_uniform(9, (77,)))
+
+
+@pytest.mark.parametrize("mode,bk,b_cols", [(0, 64, 4), (1, 128, 4), (1, 64, 2), (2, 64, 2), (2, 32, 1)])
+def test_lds_patterns_of_the_decoupled_feed(mode, bk, b_cols):
+    """The producer waves of the decoupled feed stage the 64 x 64 tile with
+    their own 256 thread indices and the direct tile's B units (columns per
+    unit as for four waves at that depth), into ring slots of the same layout."""
+    _check_tile_lds_patterns(mode, 64, 64, bk, 256, b_cols)
+
+
+def test_feed_variant_rule():
+    """Which kernel a 64 x 64 launch takes under each feed setting (pure host
+    code): the automatic rule feeds split6 launches of at most one workgroup
+    per CU with K slices of at least 512, and split3 launches of 64..256
+    workgroups with slices of at least 1024; forcing the feed picks a variant
+    for every 64 x 64 launch; other geometries never take it."""
+    prev = mm.get_feed()
+    try:
+        mm.set_feed(mm.FEED_AUTO)
+        auto = {
+            (1024, 1024, 1024, (64, 1)): (0, 2, 3),
+            (768, 768, 4096, (64, 1)): (0, 1, 3),
+            (640, 640, 3000, (64, 1)): (0, 2, 3),
+            (512, 768, 4500, (64, 1)): (0, 1, 3),
+            (512, 384, 2500, (64, 1)): (0, 0, 3),
+            (64, 512, 8192, (64, 32)): (0, 0, 0),
+            (64, 512, 8192, (64, 16)): (0, 0, 3),
+            (67, 513, 8191, (64, 16)): (0, 0, 0),
+            (4096, 4096, 4096, (256, 1)): (0, 0, 0),
+            (1024, 1024, 1024, (128, 1)): (0, 0, 0),
+        }
+        for (M, N, K, plan), want in auto.items():
+            assert tuple(mm.feed_variant(M, K, N, mode, plan) for mode in (0, 1, 2)) == want, (M, N, K, plan)
+        mm.set_feed(mm.FEED_DECOUPLED)
+        assert [mm.feed_variant(67, 8191, 513, m, (64, 16)) for m in (0, 1, 2)] == [4, 5, 6]
+        assert [mm.feed_variant(768, 4096, 768, m, (64, 1)) for m in (0, 1, 2)] == [4, 1, 3]
+        assert [mm.feed_variant(64, 8192, 512, m, (64, 32)) for m in (0, 1, 2)] == [4, 2, 3]
+        assert mm.feed_variant(1024, 1024, 1024, 1, (256, 1)) == 0
+        assert mm.feed_variant(64, 64, 64, 1, (48, 1)) == -1
+        mm.set_feed(mm.FEED_DIRECT)
+        assert mm.feed_variant(1024, 1024, 1024, 1, (64, 1)) == 0
+        with pytest.raises(ValueError):
+            mm.set_feed(-1)
+        assert mm.get_feed() == mm.FEED_DIRECT
+    finally:
+        mm.set_feed(prev)
