@@ -37,6 +37,11 @@ BF16_PEAK = 2516.6   # TFLOP/s, MI355X dense BF16 MFMA
 FP32_PEAK = 157.3    # TFLOP/s, MI355X FP32 MFMA
 HBM_PEAK = 8000.0    # GB/s
 NARROW = [(64, 512, 8192), (64, 24576, 1536), (64, 32768, 512), (128, 512, 32768), (64, 7168, 16384)]
+# Operand feed of the 64 x 64 tile, decoupled (producer and consumer waves)
+# against direct, on plans of one workgroup per CU: (M, N, K, mode, plan).
+FEED = [(768, 768, 4096, 1, (64, 1)), (1024, 1024, 1024, 1, (64, 1)), (512, 512, 8192, 1, (64, 4)),
+        (768, 768, 4096, 2, (64, 1)), (1024, 1024, 1024, 2, (64, 1)), (64, 512, 8192, 2, (64, 16)),
+        (1024, 1024, 1024, 0, (64, 1)), (64, 512, 8192, 1, (64, 32))]
 NOISE_FLOOR_PCT = 5.0  # docs/BENCHMARKS.md:2262-2280
 MIN_ROUND_MS = 5.0     # GPU time per timed round, at least
 
@@ -57,7 +62,7 @@ def main() -> int:
     ap.add_argument("--rounds", type=int, default=5, help="rotated rounds per entrant (median reported)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--only", default="", help="comma list of sections: main,narrow,grouped (default all)")
+    ap.add_argument("--only", default="", help="comma list of sections: main,narrow,feed,grouped (default all)")
     ap.add_argument("--modes", default="plain,split3,split6", help="modes timed at 4096^3")
     ap.add_argument("--grouped-classes", type=int, default=16,
                     help="shape classes the grouped autotuner searches (heaviest first)")
@@ -80,7 +85,7 @@ def main() -> int:
     dev = torch.device("cuda")
     stream = torch.cuda.current_stream()
     sptr = stream.cuda_stream
-    only = set(args.only.split(",")) if args.only else {"main", "narrow", "grouped"}
+    only = set(args.only.split(",")) if args.only else {"main", "narrow", "feed", "grouped"}
 
     def upload(x):
         return torch.from_numpy(x).to(dev)
@@ -279,11 +284,59 @@ def main() -> int:
                 alg_bytes = 4.0 * (M * K + K * N + M * N)
                 row[n] = {"tflops": flops / (ms * 1e-3) / 1e12, "ms": ms, "spread_pct": spread_pct(times[n]),
                           "plan": list(plans[n]), "hbm_gbs": alg_bytes / (ms * 1e-3) / 1e9, "errors": e}
+                # The floor of this plan's launch structure: an empty kernel and
+                # a pure streaming kernel on the same grid, plus the same reduce.
+                fl = mm.probe_stream(M, K, N, tuple(plans[n]), stream=stream)
+                fl["call_over_floor"] = ms * 1e3 / fl["stream_reduce_us"]
+                row[n]["floor"] = fl
                 log(f"{M}x{N}x{K} {n}: {ms:.4f} ms (spread {row[n]['spread_pct']:.1f}%) "
                     f"{row[n]['tflops']:.1f} TFLOP/s {row[n]['hbm_gbs']:.0f} GB/s plan {plans[n]} "
-                    f"ratio {e['bound_ratio']:.3g}")
+                    f"ratio {e['bound_ratio']:.3g}; floor {fl['stream_reduce_us']:.1f} us (empty kernel "
+                    f"{fl['empty_us']:.1f}, streaming {fl['stream_us']:.1f})")
             shapes.append(row)
             del dA, dB, dC
+        torch.cuda.empty_cache()
+
+    feed = []
+    if "feed" in only:
+        # The same plan under both feeds, in the same rotated rounds; the
+        # feed setting is read at each launch, so each entrant sets it.
+        prev = mm.get_feed()
+        for i, (M, N, K, mode, plan) in enumerate(FEED):
+            dA, dB = upload(inputs.uniform(5, (M, K))), upload(inputs.uniform(6, (K, N)))
+            dC = torch.empty((M, N), dtype=torch.float32, device=dev)
+            call = make_call(dA, dB, dC, M, K, N, mode, plan)
+
+            def entrant(setting, call=call):
+                def f():
+                    mm.set_feed(setting)
+                    call()
+                return f
+            names = {"direct": mm.FEED_DIRECT, "decoupled": mm.FEED_DECOUPLED}
+            times = rounds({n: entrant(v) for n, v in names.items()}, max(args.steps, 20), i)
+            outs = {}
+            for n, v in names.items():
+                mm.set_feed(v)
+                call()
+                torch.cuda.synchronize()
+                outs[n] = dC.clone()
+            e = errors_torch(dA, dB, dC, K)
+            mm.set_feed(mm.FEED_AUTO)
+            row = {"shape_mnk": [M, N, K], "mode": ["plain", "split3", "split6"][mode], "plan": list(plan),
+                   "decoupled_variant": None, "auto_takes_decoupled": mm.feed_variant(M, K, N, mode, plan) > 0,
+                   "identical": bool(torch.equal(outs["direct"], outs["decoupled"])), "errors": e}
+            mm.set_feed(mm.FEED_DECOUPLED)
+            row["decoupled_variant"] = mm.feed_variant(M, K, N, mode, plan)
+            for n in names:
+                ms = float(np.median(times[n]))
+                row[n] = {"ms": ms, "spread_pct": spread_pct(times[n]), "tflops": 2.0 * M * N * K / (ms * 1e-3) / 1e12}
+            row["speedup"] = row["direct"]["ms"] / row["decoupled"]["ms"]
+            log(f"feed {M}x{N}x{K} {row['mode']} {plan}: direct {row['direct']['ms']*1e3:.1f} us, decoupled "
+                f"{row['decoupled']['ms']*1e3:.1f} us ({row['speedup']:.3f}x, variant {row['decoupled_variant']}, "
+                f"auto {'decoupled' if row['auto_takes_decoupled'] else 'direct'}), identical {row['identical']}")
+            feed.append(row)
+            del dA, dB, dC
+        mm.set_feed(prev)
         torch.cuda.empty_cache()
 
     grouped = None
@@ -349,6 +402,8 @@ def main() -> int:
                                   "frac": out["main"]["plain"]["tflops"] / FP32_PEAK}
     if shapes:
         line["narrow_m"] = shapes
+    if feed:
+        line["feed"] = feed
     if grouped:
         line["grouped"] = grouped
     if cpu:

@@ -338,6 +338,20 @@ int maxmulsk_get_feed(void);
  * flight differ per variant), -1 = invalid plan. */
 int maxmulsk_feed_variant(size_t M, size_t K, size_t N, maxmulsk_mode mode, const maxmulsk_plan* plan);
 
+/* Streaming-floor probe, no counterpart in the reference: for a direct-tile
+ * plan (not the packed 256 x 256 path) of an M x K x N GEMM, times on
+ * `stream` (NULL: the legacy default stream) of the current device, per call
+ * over `calls` back-to-back launches: us[0] an empty kernel on the GEMM's grid
+ * (the launch floor), us[1] a kernel on the same grid that reads exactly each
+ * workgroup's A rows and B columns over its K slice and writes its block of
+ * split-K partials, with no arithmetic but the loads' sum, and us[2] that
+ * kernel followed by the library's split-K reduce (as us[1] when the plan has
+ * no split). Buffers are the probe's own; the call is synchronous. K and N
+ * must be multiples of 4. What a call with that plan's launches and bytes
+ * costs without staging, splitting or MFMA work. */
+maxmulsk_status maxmulsk_probe_stream(size_t M, size_t K, size_t N, const maxmulsk_plan* plan, int calls,
+                                      void* stream, double* us);
+
 /* Number of usable gfx950 devices (0 when none). */
 int maxmulsk_device_count(void);
 /* Text of the last error on this thread (never NULL). */

@@ -71,6 +71,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_sgemm_ws": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), _P, _SIZE, _P]),
     "maxmulsk_sgemm_noalloc": (C.c_int, [_P, _P, _P, _SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan), _P]),
     "maxmulsk_probe_mfma": (C.c_int, [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
+    "maxmulsk_probe_stream": (C.c_int, [_SIZE, _SIZE, _SIZE, C.POINTER(Plan), C.c_int, _P, C.POINTER(C.c_double)]),
     "maxmulsk_set_feed": (C.c_int, [C.c_int]),
     "maxmulsk_get_feed": (C.c_int, []),
     "maxmulsk_feed_variant": (C.c_int, [_SIZE, _SIZE, _SIZE, C.c_int, C.POINTER(Plan)]),
@@ -231,6 +232,15 @@ def probe_mfma(mode: int, random_operands: bool, iterations: int = 20000) -> flo
     out = C.c_double()
     check(lib().maxmulsk_probe_mfma(mode, int(random_operands), iterations, C.byref(out)))
     return out.value
+
+
+def probe_stream(M: int, K: int, N: int, plan: tuple[int, int], calls: int = 400, stream: Any = None) -> dict:
+    """maxmulsk_probe_stream: per-call us of an empty kernel on the plan's grid,
+    a streaming kernel over each workgroup's operand bytes, and that kernel plus
+    the split-K reduce, on `stream` (None: the legacy default stream)."""
+    us = (C.c_double * 3)()
+    check(lib().maxmulsk_probe_stream(M, K, N, C.byref(Plan(*plan)), calls, _stream_ptr(stream), us))
+    return {"empty_us": us[0], "stream_us": us[1], "stream_reduce_us": us[2]}
 
 
 FEED_DIRECT = 0     # MAXMULSK_FEED_DIRECT
