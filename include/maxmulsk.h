@@ -313,29 +313,34 @@ void maxmulsk_phase_counters_reset(void);
 maxmulsk_status maxmulsk_probe_mfma(int mode, int random_operands, int iterations, double* tflops);
 
 /* Operand feed of the 64 x 64 tile (plans with tile 64, and narrow-M plans
- * whose tile shrinks to 64 x 64). MAXMULSK_FEED_DIRECT: each wave stages,
- * splits and multiplies (the direct tile). MAXMULSK_FEED_DECOUPLED: every such
- * launch runs four producer waves that stage and split k-blocks into an LDS
- * ring and four consumer waves that only multiply and write C, handing slots
- * over through ready/consumed counters. MAXMULSK_FEED_AUTO (the default): the
- * decoupled feed where it measured faster, i.e. launches of at most one
- * workgroup per CU with K slices of at least 512 (split6) or of 64 to 256
- * workgroups with slices of at least 1024 (split3); the direct tile elsewhere. Both feeds give the same C (same K slices, products and
- * order), except that an exact -0 may come out as +0. Counterpart of the
- * reference's Trio engine, one compute thread fed by separate packing threads
- * through rings (sme/trio/trio.hpp:10-24, sme/trio/trio_sme.cpp:274-334). The
- * setting is process-wide; the first use reads MAXMULSK_FEED=direct|
- * decoupled (anything else: auto). maxmulsk_set_feed returns the previous
- * setting, or -1 for an unknown value (nothing changes). */
+ * whose tile shrinks to 64 x 64) and of the split modes' 128 x 128 tile.
+ * MAXMULSK_FEED_DIRECT: each wave stages, splits and multiplies (the direct
+ * tile). MAXMULSK_FEED_DECOUPLED: every such launch runs producer waves that
+ * stage and split k-blocks into an LDS ring and consumer waves that only
+ * multiply and write C, handing slots over through ready/consumed counters.
+ * MAXMULSK_FEED_AUTO (the default): the decoupled feed where it measured
+ * faster, on launches of at most one workgroup per CU: 64 x 64 split6 with K
+ * slices of at least 512, 64 x 64 split3 on 64..256 workgroups with slices of
+ * at least 1024, and 128 x 128 split3/split6 whose tiles and k-blocks are all
+ * full (M, N multiples of 128, K and slices multiples of 64, slices of at
+ * least 1024); the direct tile elsewhere. Both feeds give the same C (same K
+ * slices, products and order), except that an exact -0 may come out as +0.
+ * Grouped launches keep their own kernel. Counterpart of the reference's Trio
+ * engine, one compute thread fed by separate packing threads through rings
+ * (sme/trio/trio.hpp:10-24, sme/trio/trio_sme.cpp:274-334). The setting is
+ * process-wide and read at each launch; the first use reads
+ * MAXMULSK_FEED=direct|decoupled (anything else: auto). maxmulsk_set_feed
+ * returns the previous setting, or -1 for an unknown value (nothing
+ * changes). */
 #define MAXMULSK_FEED_DIRECT 0
 #define MAXMULSK_FEED_DECOUPLED 1
 #define MAXMULSK_FEED_AUTO 2
 int maxmulsk_set_feed(int feed);
 int maxmulsk_get_feed(void);
 /* Kernel a call with this shape, mode and plan launches under the current
- * feed setting: 0 = the direct tile (or a geometry other than 64 x 64),
- * 1..6 = a decoupled-feed variant (k-block depth, ring slots and loads in
- * flight differ per variant), -1 = invalid plan. */
+ * feed setting: 0 = the direct tile (or the packed path), 1..6 = a 64 x 64
+ * decoupled-feed variant, 7..8 = a 128 x 128 one (k-block depth, ring slots,
+ * waves and loads in flight differ per variant), -1 = invalid plan. */
 int maxmulsk_feed_variant(size_t M, size_t K, size_t N, maxmulsk_mode mode, const maxmulsk_plan* plan);
 
 /* Streaming-floor probe, no counterpart in the reference: for a direct-tile

@@ -261,7 +261,7 @@ def get_feed() -> int:
 
 
 def feed_variant(M: int, K: int, N: int, mode: int, plan: tuple[int, int]) -> int:
-    """maxmulsk_feed_variant: 0 = direct tile, 1..6 = decoupled-feed variant, -1 = invalid plan."""
+    """maxmulsk_feed_variant: 0 = direct tile, 1..8 = decoupled-feed variant, -1 = invalid plan."""
     return lib().maxmulsk_feed_variant(M, K, N, mode, C.byref(Plan(*plan)))
 
 

@@ -193,6 +193,6 @@ extern "C" int maxmulsk_feed_variant(size_t M, size_t K, size_t N, maxmulsk_mode
     if (!plan) return -1;
     Resolved r{};
     if (!resolve(M, K, N, (int)mode, *plan, &r)) return -1;
-    if (r.cfg != 0) return 0;
-    return feed_variant((int)mode, (long long)r.tiles_m * r.tiles_n * r.splits, r.k_len, feed_mode());
+    return feed_variant((int)mode, r.cfg, (long long)M, (long long)N, (long long)K,
+                        (long long)r.tiles_m * r.tiles_n * r.splits, r.k_len, feed_mode());
 }

@@ -75,14 +75,20 @@ __device__ __forceinline__ void signal(unsigned* cnt) {
     if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <int MODE, int BK, int NS, int PFR>
+// A BM x BN tile with a CWM x CWN wave layout of consumers (the direct tile's
+// MFMA layout) and a PWM x PWN layout of producers (the direct tile's staging
+// map for that many threads); both sides address the same fragment-major slot
+// layout, which depends only on the mode, BM, BN and BK.
+template <int MODE, int BM, int BN, int BK, int NS, int PFR, int CWM, int CWN, int PWM, int PWN>
 struct Feed {
-    using Cons = Tile<MODE, 64, 64, BK, 2, 2, 1>;
-    using Prod = Tile<MODE, 64, 64, BK, 2, 2, PFR>;
+    using Cons = Tile<MODE, BM, BN, BK, CWM, CWN, 1>;
+    using Prod = Tile<MODE, BM, BN, BK, PWM, PWN, PFR>;
+    static constexpr int CT = Cons::NT, PT = Prod::NT;  // consumer, producer threads
+    static constexpr unsigned CWAVES = CT / 64, PWAVES = PT / 64;
     static constexpr int STAGE = Cons::STAGE;
     static constexpr int RING = NS * STAGE;
     static_assert(Prod::STAGE == STAGE, "one slot layout");
-    static_assert(Cons::NT == 256, "four waves on each side");
+    static_assert(CT + PT <= 1024, "one workgroup");
     static_assert(RING <= 160 * 1024 - 256, "LDS budget of one CU");
 
     struct Shared {
@@ -105,7 +111,7 @@ struct Feed {
         if (j < nkb) {
             const int slot = (int)(j % NS);
             const unsigned round = (unsigned)(j / NS);
-            ok = round == 0 || wait_count(&sh.freed[slot], 4u * round, &sh.abort_flag, limit);
+            ok = round == 0 || wait_count(&sh.freed[slot], CWAVES * round, &sh.abort_flag, limit);
             if (ok) {
                 t.template store_as<S>(ptid, ring + slot * STAGE);
                 signal(&sh.ready[slot]);
@@ -124,7 +130,7 @@ struct Feed {
                                                     long long nkb, unsigned limit) {
         if (nkb <= 0) return;
         Prod t;
-        const int ptid = (int)threadIdx.x - 256;
+        const int ptid = (int)threadIdx.x - CT;
         auto fetch = [&](auto set, long long j) {
             const long long k0 = kbeg + min(j, nkb - 1) * BK;
             t.template load_as<decltype(set)::value>(ptid, p.A, p.B, p.M, p.N, p.lda, p.ldb, m0, n0, k0, kend,
@@ -150,12 +156,16 @@ struct Feed {
                                                     long long m0, long long n0, long long nkb, unsigned limit) {
         Cons t;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) t.acc[0][0][e] = 0.f;
+        for (int i = 0; i < Cons::TM; ++i)
+#pragma unroll
+            for (int j = 0; j < Cons::TN; ++j)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) t.acc[i][j][e] = 0.f;
         bool ok = true;
         for (long long j = 0; j < nkb; ++j) {
             const int slot = (int)(j % NS);
             const unsigned round = (unsigned)(j / NS);
-            if (!wait_count(&sh.ready[slot], 4u * (round + 1), &sh.abort_flag, limit)) {
+            if (!wait_count(&sh.ready[slot], PWAVES * (round + 1), &sh.abort_flag, limit)) {
                 ok = false;
                 break;
             }
@@ -164,7 +174,11 @@ struct Feed {
         }
         if (!ok) {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) t.acc[0][0][e] = __builtin_nanf("");
+            for (int i = 0; i < Cons::TM; ++i)
+#pragma unroll
+                for (int j = 0; j < Cons::TN; ++j)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) t.acc[i][j][e] = __builtin_nanf("");
         }
         t.epilogue(dst, p.M, p.N, p.ldc, m0, n0);
     }
@@ -172,9 +186,9 @@ struct Feed {
 
 }  // namespace
 
-template <int MODE, int BK, int NS, int PFR>
-__global__ __launch_bounds__(512) void gemm_feed_kernel(GemmParams p, unsigned spin_limit) {
-    using F = Feed<MODE, BK, NS, PFR>;
+template <int MODE, int BM, int BN, int BK, int NS, int PFR, int CWM, int CWN, int PWM, int PWN>
+__global__ __launch_bounds__(64 * (CWM * CWN + PWM * PWN)) void gemm_feed_kernel(GemmParams p, unsigned spin_limit) {
+    using F = Feed<MODE, BM, BN, BK, NS, PFR, CWM, CWN, PWM, PWN>;
     __shared__ __attribute__((aligned(16))) char ring[F::RING];
     __shared__ typename F::Shared sh;
     if (threadIdx.x < NS) {
@@ -191,10 +205,10 @@ __global__ __launch_bounds__(512) void gemm_feed_kernel(GemmParams p, unsigned s
     grouped_tile(bid - split * ntiles, p.tiles_m, p.tiles_n, tm, tn);
     const long long kbeg = split * p.k_len;
     const long long kend = min(p.K, kbeg + p.k_len);
-    const long long m0 = (long long)tm * 64, n0 = (long long)tn * 64;
+    const long long m0 = (long long)tm * BM, n0 = (long long)tn * BN;
     const long long nkb = (kend - kbeg + BK - 1) / BK;
-    const bool full_mn = (m0 + 64 <= p.M) && (n0 + 64 <= p.N);
-    if (threadIdx.x < 256) {
+    const bool full_mn = (m0 + BM <= p.M) && (n0 + BN <= p.N);
+    if (threadIdx.x < F::CT) {
         F::consumer(ring, sh, p.C + split * p.c_split_stride, p, m0, n0, nkb, spin_limit);
     } else if (full_mn && (kend - kbeg) % BK == 0) {
         F::template producer<true>(ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, spin_limit);
@@ -222,12 +236,13 @@ int feed_from_env() {
     return 2;
 }
 
-template <int MODE, int BK, int NS, int PFR>
+template <int MODE, int BK, int NS, int PFR, int BM = 64, int BN = 64, int CWM = 2, int CWN = 2, int PWM = 2,
+          int PWN = 2>
 hipError_t launch_feed_cfg(const GemmParams& p, hipStream_t stream) {
     const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
     if (grid <= 0 || grid > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((gemm_feed_kernel<MODE, BK, NS, PFR>), dim3((unsigned)grid), dim3(512), 0, stream, p,
-                       spin_limit());
+    hipLaunchKernelGGL((gemm_feed_kernel<MODE, BM, BN, BK, NS, PFR, CWM, CWN, PWM, PWN>), dim3((unsigned)grid),
+                       dim3(64 * (CWM * CWN + PWM * PWN)), 0, stream, p, spin_limit());
     return hipGetLastError();
 }
 
@@ -260,6 +275,17 @@ void set_feed_mode(int v) { g_feed.store(v, std::memory_order_relaxed); }
 //     38.9 us with 1, 83 x 4819 x 6314 (64,1) 1.12x), and is even or 2-7%
 //     slower on smaller grids and shorter slices. Variant 1 (two blocks of
 //     loads in flight) for slices of 4096 or more, 2 below.
+//   7 split3 128 x 128, BK 64, NS 2, PFR 1, eight consumer and eight producer
+//     waves, and 8 split6 128 x 128, BK 32, NS 2, PFR 2, eight consumer and
+//     four producer waves: grids of at most one workgroup per CU whose tiles
+//     and k-blocks are all full (M, N multiples of 128; K and the slice
+//     multiples of 64), slices of at least 1024: 2048^3 (128,1) split3 61.7 ->
+//     59.7 us, 512 x 4096 x 4096 94.5 -> 83.7 us, 1152 x 1152 x 8192 184.2 ->
+//     161.6 us; split6 1.03-1.10x. Ragged tiles or a partial last k-block put
+//     the element-wise edge loads on the producers alone: 3-18% slower
+//     (1920 x 1920 x 2000, 1000 x 1100 x 3000), so those stay direct, as do
+//     grids of more than one workgroup per CU (4096 x 4096 x 1024: 0.94x).
+//     Plain keeps the direct 128 x 128 tile.
 //   4 plain BK 64, NS 2, PFR 2; 5 split3 BK 64, NS 2, PFR 2; 6 split6 BK 32,
 //     NS 4, PFR 2: only when the feed is forced. Plain was 3-40% slower on
 //     small grids with every variant, and on grids of more than one
@@ -268,9 +294,19 @@ void set_feed_mode(int v) { g_feed.store(v, std::memory_order_relaxed); }
 // Slices shorter than the k-block depth are zero-padded, as in the deep
 // direct variants: results are those of the direct tile bit for bit, except
 // that an exact -0 may come out as +0.
-int feed_variant(int mode, long long grid, long long k_len, int setting) {
+int feed_variant(int mode, int cfg, long long M, long long N, long long K, long long grid, long long k_len,
+                 int setting) {
     if (setting == 0 || mode < 0 || mode > 2) return 0;
     const bool small_grid = grid > 0 && grid <= kDeepMaxGrid;
+    if (cfg == 3) {
+        // 128 x 128 (split modes): only where every tile and k-block is full;
+        // plain keeps the direct tile.
+        if (mode == 0) return 0;
+        if (setting == 1) return mode == 1 ? 7 : 8;
+        const bool whole = M % 128 == 0 && N % 128 == 0 && K % 64 == 0 && k_len % 64 == 0;
+        return (small_grid && whole && k_len >= 1024) ? (mode == 1 ? 7 : 8) : 0;
+    }
+    if (cfg != 0) return 0;
     if (setting == 2) {  // measured rule
         if (!small_grid) return 0;
         if (mode == 1) return (grid >= 64 && k_len >= 1024) ? (k_len >= 4096 ? 1 : 2) : 0;
@@ -289,6 +325,8 @@ hipError_t launch_gemm_feed(int variant, const GemmParams& p, hipStream_t stream
         case 4: return launch_feed_cfg<0, 64, 2, 2>(p, stream);
         case 5: return launch_feed_cfg<1, 64, 2, 2>(p, stream);
         case 6: return launch_feed_cfg<2, 32, 4, 2>(p, stream);
+        case 7: return launch_feed_cfg<1, 64, 2, 1, 128, 128, 2, 4, 2, 4>(p, stream);
+        case 8: return launch_feed_cfg<2, 32, 2, 2, 128, 128, 2, 4, 2, 2>(p, stream);
         default: return hipErrorInvalidValue;
     }
 }

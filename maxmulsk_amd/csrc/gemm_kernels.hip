@@ -68,7 +68,8 @@ template <int MODE>
 static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) {
     switch (cfg) {
         case 0:
-            if (const int fv = feed_variant(MODE, (long long)p.tiles_m * p.tiles_n * p.splits, p.k_len, feed_mode()))
+            if (const int fv = feed_variant(MODE, 0, p.M, p.N, p.K, (long long)p.tiles_m * p.tiles_n * p.splits,
+                                            p.k_len, feed_mode()))
                 return launch_gemm_feed(fv, p, stream);
             if constexpr (kDeepBK[MODE] > 0) {
                 const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
@@ -80,6 +81,9 @@ static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) 
         case 1: return launch_one<MODE, 1>(p, stream);
         case 2: return launch_one<MODE, 2>(p, stream);
         case 3:
+            if (const int fv = feed_variant(MODE, 3, p.M, p.N, p.K, (long long)p.tiles_m * p.tiles_n * p.splits,
+                                            p.k_len, feed_mode()))
+                return launch_gemm_feed(fv, p, stream);
             if constexpr (kDeep128BK[MODE] > 0) {
                 const long long grid = (long long)p.tiles_m * p.tiles_n * p.splits;
                 if (grid > 0 && grid <= kDeepMaxGrid) {

@@ -86,17 +86,20 @@ struct GemmParams {
 
 hipError_t launch_gemm(int mode, int cfg, const GemmParams& p, hipStream_t stream);
 
-// Decoupled producer/consumer feed of geometry 0 (feed.hip): four waves stage
-// and split operands into an LDS ring, four others only multiply; same K
-// slices, products and order as the direct 64 x 64 tile. feed_mode(): 0 =
-// direct tile kernels only, 1 = every geometry-0 launch takes the feed, 2 =
-// the feed where it measured faster (the default). MAXMULSK_FEED=direct|
+// Decoupled producer/consumer feed of geometries 0 and 3 (feed.hip): producer
+// waves stage and split operands into an LDS ring, consumer waves only
+// multiply; same K slices, products and order as the direct tile. feed_mode():
+// 0 = direct tile kernels only, 1 = every 64 x 64 launch and every split-mode
+// 128 x 128 launch takes the feed, 2 = the feed where it measured faster (the
+// default). MAXMULSK_FEED=direct|
 // decoupled sets 0 or 1 at first use; set_feed_mode overrides.
 int feed_mode();
 void set_feed_mode(int v);
-// Feed variant of a geometry-0 launch of `grid` workgroups and K slices of
-// k_len under a feed setting; 0 = the direct tile.
-int feed_variant(int mode, long long grid, long long k_len, int setting);
+// Feed variant of a launch of geometry cfg (0: 64 x 64, 3: 128 x 128) for an
+// M x N x K product on `grid` workgroups with K slices of k_len, under a feed
+// setting; 0 = the direct tile.
+int feed_variant(int mode, int cfg, long long M, long long N, long long K, long long grid, long long k_len,
+                 int setting);
 hipError_t launch_gemm_feed(int variant, const GemmParams& p, hipStream_t stream);
 
 // ---- packed path (256 x 256 geometry) -------------------------------------------

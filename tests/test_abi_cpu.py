@@ -386,20 +386,25 @@ def test_product_input_generator_matches_oracle():
     assert np.array_equal(inputs.log_uniform(9, (77,)), orc.log_uniform(9, (77,)))
 
 
-@pytest.mark.parametrize("mode,bk,b_cols", [(0, 64, 4), (1, 128, 4), (1, 64, 2), (2, 64, 2), (2, 32, 1)])
-def test_lds_patterns_of_the_decoupled_feed(mode, bk, b_cols):
-    """The producer waves of the decoupled feed stage the 64 x 64 tile with
-    their own 256 thread indices and the direct tile's B units (columns per
-    unit as for four waves at that depth), into ring slots of the same layout."""
-    _check_tile_lds_patterns(mode, 64, 64, bk, 256, b_cols)
+@pytest.mark.parametrize("mode,rows,bk,threads,b_cols", [
+    (0, 64, 64, 256, 4), (1, 64, 128, 256, 4), (1, 64, 64, 256, 2), (2, 64, 64, 256, 2), (2, 64, 32, 256, 1),
+    (1, 128, 64, 512, 2), (2, 128, 32, 256, 2)])
+def test_lds_patterns_of_the_decoupled_feed(mode, rows, bk, threads, b_cols):
+    """The producer waves of the decoupled feed stage the tile (64 x 64, or
+    128 x 128 in the split modes) with their own thread indices and the direct
+    tile's staging units for that many threads (B columns per unit as for that
+    wave count and depth), into ring slots of the direct tile's layout."""
+    _check_tile_lds_patterns(mode, rows, rows, bk, threads, b_cols)
 
 
 def test_feed_variant_rule():
-    """Which kernel a 64 x 64 launch takes under each feed setting (pure host
-    code): the automatic rule feeds split6 launches of at most one workgroup
-    per CU with K slices of at least 512, and split3 launches of 64..256
-    workgroups with slices of at least 1024; forcing the feed picks a variant
-    for every 64 x 64 launch; other geometries never take it."""
+    """Which kernel a launch takes under each feed setting (pure host code):
+    the automatic rule feeds 64 x 64 split6 launches of at most one workgroup
+    per CU with K slices of at least 512, 64 x 64 split3 launches of 64..256
+    workgroups with slices of at least 1024, and 128 x 128 split-mode launches
+    of at most one workgroup per CU whose tiles and k-blocks are all full;
+    forcing the feed picks a variant for every 64 x 64 launch and every
+    split-mode 128 x 128 launch; other geometries never take it."""
     prev = mm.get_feed()
     try:
         mm.set_feed(mm.FEED_AUTO)
@@ -413,7 +418,12 @@ def test_feed_variant_rule():
             (64, 512, 8192, (64, 16)): (0, 0, 3),
             (67, 513, 8191, (64, 16)): (0, 0, 0),
             (4096, 4096, 4096, (256, 1)): (0, 0, 0),
-            (1024, 1024, 1024, (128, 1)): (0, 0, 0),
+            (1024, 1024, 1024, (128, 1)): (0, 7, 8),        # 128 x 128, full tiles and k-blocks
+            (2048, 2048, 2048, (128, 1)): (0, 7, 8),
+            (1920, 1920, 2000, (128, 1)): (0, 0, 0),        # K not a multiple of 64
+            (1000, 1100, 3000, (128, 1)): (0, 0, 0),        # ragged tiles
+            (2048, 2048, 512, (128, 1)): (0, 0, 0),         # slices of 512
+            (4096, 4096, 1024, (128, 1)): (0, 0, 0),        # 1024 workgroups
         }
         for (M, N, K, plan), want in auto.items():
             assert tuple(mm.feed_variant(M, K, N, mode, plan) for mode in (0, 1, 2)) == want, (M, N, K, plan)
@@ -422,6 +432,8 @@ def test_feed_variant_rule():
         assert [mm.feed_variant(768, 4096, 768, m, (64, 1)) for m in (0, 1, 2)] == [4, 1, 3]
         assert [mm.feed_variant(64, 8192, 512, m, (64, 32)) for m in (0, 1, 2)] == [4, 2, 3]
         assert mm.feed_variant(1024, 1024, 1024, 1, (256, 1)) == 0
+        assert [mm.feed_variant(1000, 3000, 1100, m, (128, 1)) for m in (0, 1, 2)] == [0, 7, 8]
+        assert [mm.feed_variant(4096, 1024, 4096, m, (128, 1)) for m in (0, 1, 2)] == [0, 7, 8]
         assert mm.feed_variant(64, 64, 64, 1, (48, 1)) == -1
         mm.set_feed(mm.FEED_DIRECT)
         assert mm.feed_variant(1024, 1024, 1024, 1, (64, 1)) == 0

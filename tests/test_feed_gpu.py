@@ -1,10 +1,11 @@
-"""The decoupled producer/consumer feed of the 64 x 64 tile (the GPU analogue
-of the reference's Trio engine, sme/trio/trio_sme.cpp:274-334, whose result is
-bit-identical to the v5.1 kernel it feeds, README.md "MaxMulSK Trio"): every
-variant gives the direct tile's C bit for bit (-0 may become +0) on full and
-ragged tiles, with and without split-K; exact on integer inputs; the automatic
-rule launches it where it measured faster; a handshake that runs out of polls
-ends the kernel with NaN in C instead of hanging."""
+"""The decoupled producer/consumer feed of the 64 x 64 tile and of the split
+modes' 128 x 128 tile (the GPU analogue of the reference's Trio engine,
+sme/trio/trio_sme.cpp:274-334, whose result is bit-identical to the v5.1 kernel
+it feeds, README.md "MaxMulSK Trio"): every variant gives the direct tile's C
+bit for bit (-0 may become +0) on full and ragged tiles, with and without
+split-K; exact on integer inputs; the automatic rule launches it where it
+measured faster; a handshake that runs out of polls ends the kernel with NaN
+in C instead of hanging."""
 import json
 import os
 import subprocess
@@ -65,14 +66,45 @@ def test_forced_feed_matches_direct_tile(feed_setting, mode, M, N, K, plan):
     assert np.array_equal(Cf, Cd), (mode, M, N, K, plan, v, float(np.abs(Cf - Cd).max()))
 
 
+# 128 x 128 tiles of the split modes: full tiles and k-blocks (the automatic
+# rule's case), ragged M, N and K, a partial last k-block, split-K.
+CASES_128 = [
+    (256, 256, 2048, (128, 1)),
+    (384, 256, 640, (128, 2)),
+    (200, 300, 1000, (128, 1)),
+    (130, 260, 4099, (128, 4)),
+    (1000, 1100, 3000, (128, 1)),
+    (129, 129, 33, (128, 1)),
+]
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("M,N,K,plan", CASES_128)
+def test_forced_feed_128_matches_direct_tile(feed_setting, mode, M, N, K, plan):
+    A = orc.uniform(41 + M, (M, K))
+    B = orc.uniform(42 + N, (K, N))
+    mm.set_feed(mm.FEED_DIRECT)
+    Cd = gpu_gemm(A, B, mode=mode, plan=plan)
+    mm.set_feed(mm.FEED_DECOUPLED)
+    v = mm.feed_variant(M, K, N, mode, plan)
+    assert v == (7 if mode == 1 else 8), v
+    Cf = gpu_gemm(A, B, mode=mode, plan=plan)
+    assert not np.isnan(Cf).any()
+    assert np.array_equal(Cf, Cd), (mode, M, N, K, plan, float(np.abs(Cf - Cd).max()))
+    # plain mode keeps the direct 128 x 128 tile even when the feed is forced
+    assert mm.feed_variant(M, K, N, 0, plan) == 0
+
+
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_forced_feed_exact_on_integers(feed_setting, mode):
     """Small integers: every product and partial sum is exact in FP32 (and in
     BF16 hi terms), so C must equal the integer product whatever the order."""
     rng = np.random.default_rng(5 + mode)
     mm.set_feed(mm.FEED_DECOUPLED)
-    for M, N, K, plan in ((96, 160, 1100, (64, 1)), (130, 70, 4099, (64, 8)), (256, 256, 2048, (64, 1)),
-                          (1100, 300, 700, (64, 2))):
+    cases = [(96, 160, 1100, (64, 1)), (130, 70, 4099, (64, 8)), (256, 256, 2048, (64, 1)), (1100, 300, 700, (64, 2))]
+    if mode != 0:
+        cases += [(256, 384, 2048, (128, 1)), (200, 300, 1000, (128, 2))]
+    for M, N, K, plan in cases:
         A = rng.integers(-8, 9, (M, K)).astype(np.float32)
         B = rng.integers(-8, 9, (K, N)).astype(np.float32)
         assert mm.feed_variant(M, K, N, mode, plan) > 0
@@ -92,7 +124,9 @@ AUTO = [
     ((67, 513, 8191, (64, 16)), {0: 0, 1: 0, 2: 0}),      # 288 workgroups: direct
     ((64, 7168, 16384, (64, 8)), {0: 0, 1: 0, 2: 0}),
     ((4096, 4096, 4096, (256, 1)), {0: 0, 1: 0, 2: 0}),   # packed path
-    ((1024, 1024, 1024, (128, 1)), {0: 0, 1: 0, 2: 0}),   # 128 x 128 tile
+    ((1024, 1024, 1024, (128, 1)), {0: 0, 1: 7, 2: 8}),   # 128 x 128, full tiles and k-blocks
+    ((1920, 1920, 2000, (128, 1)), {0: 0, 1: 0, 2: 0}),   # partial last k-block
+    ((1000, 1100, 3000, (128, 1)), {0: 0, 1: 0, 2: 0}),   # ragged tiles
 ]
 
 
@@ -103,7 +137,8 @@ def test_auto_rule_and_results(feed_setting):
             assert mm.feed_variant(M, K, N, mode, plan) == v, (M, N, K, plan, mode)
     # an automatically fed launch gives the direct tile's C, inside the FP32 bound
     for mode, (M, N, K, plan) in ((1, (640, 640, 3000, (64, 1))), (1, (512, 768, 4500, (64, 1))),
-                                  (2, (200, 300, 1200, (64, 2))), (2, (128, 1024, 4096, (64, 8)))):
+                                  (2, (200, 300, 1200, (64, 2))), (2, (128, 1024, 4096, (64, 8))),
+                                  (1, (512, 384, 2048, (128, 1))), (2, (384, 512, 1024, (128, 1)))):
         A = orc.uniform(21, (M, K))
         B = orc.uniform(22, (K, N))
         mm.set_feed(mm.FEED_AUTO)
