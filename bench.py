@@ -40,9 +40,10 @@ NARROW = [(64, 512, 8192), (64, 24576, 1536), (64, 32768, 512), (128, 512, 32768
 # Operand feed of the 64 x 64 and 128 x 128 tiles, decoupled (producer and
 # consumer waves) against direct, on plans of one workgroup per CU: (M, N, K,
 # mode, plan). The last three are plans the automatic feed leaves direct.
-FEED = [(768, 768, 4096, 1, (64, 1)), (1024, 1024, 1024, 1, (64, 1)), (512, 512, 8192, 1, (64, 4)),
-        (768, 768, 4096, 2, (64, 1)), (1024, 1024, 1024, 2, (64, 1)), (64, 512, 8192, 2, (64, 16)),
-        (2048, 2048, 2048, 1, (128, 1)), (512, 4096, 4096, 1, (128, 1)), (1152, 1152, 8192, 2, (128, 1)),
+FEED = [(768, 768, 4096, 1, (64, 1)), (1024, 1024, 1024, 1, (64, 1)), (1000, 1000, 1000, 1, (64, 1)),
+        (512, 512, 8192, 1, (64, 4)), (768, 768, 4096, 2, (64, 1)), (1024, 1024, 1024, 2, (64, 1)),
+        (700, 900, 3000, 2, (64, 1)), (64, 512, 8192, 2, (64, 16)), (2048, 2048, 2048, 1, (128, 1)),
+        (512, 4096, 4096, 1, (128, 1)), (1000, 1000, 4000, 1, (128, 1)), (1152, 1152, 8192, 2, (128, 1)),
         (1024, 1024, 1024, 0, (64, 1)), (64, 512, 8192, 1, (64, 32)), (1920, 1920, 2000, 1, (128, 1))]
 NOISE_FLOOR_PCT = 5.0  # docs/BENCHMARKS.md:2262-2280
 MIN_ROUND_MS = 5.0     # GPU time per timed round, at least

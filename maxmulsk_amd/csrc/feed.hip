@@ -97,18 +97,36 @@ struct Feed {
         unsigned abort_flag;
     };
 
-    // Producer step for k-block j from register set S (when j < nkb), then
+    // How a producer step loads a k-block: every unit a plain vector load
+    // (full tiles), one clamped vector load per unit (k-block inside K, tile
+    // past M or N, N a multiple of the B unit width), or the tile's general
+    // path (vector loads inside, element-wise across an edge).
+    enum { kLoadFull = 0, kLoadEdge = 1, kLoadGeneral = 2 };
+
+    template <int S, int POL>
+    __device__ __forceinline__ static void load_block(Prod& t, int ptid, const GemmParams& p, long long m0,
+                                                      long long n0, long long k0, long long kend, bool full_mn) {
+        if constexpr (POL == kLoadFull)
+            t.template load_as<S>(ptid, p.A, p.B, p.M, p.N, p.lda, p.ldb, m0, n0, k0, kend, true);
+        else if constexpr (POL == kLoadEdge)
+            t.template load_edge_as<S>(ptid, p.A, p.B, p.M, p.N, p.lda, p.ldb, m0, n0, k0);
+        else
+            t.template load_as<S>(ptid, p.A, p.B, p.M, p.N, p.lda, p.ldb, m0, n0, k0, kend,
+                                  full_mn && k0 + BK <= kend);
+    }
+
+    // Producer step for k-block j from register set S (when j < nloop), then
     // the loads of k-block j + PFR into the same set. The loads are issued on
-    // every step, past the last block as repeats of it (the block index is
-    // clamped), so each step's loads are unconditional: with conditional
+    // every step, past the loop's last block as repeats of it (the block index
+    // is clamped), so each step's loads are unconditional: with conditional
     // loads the compiler's wait counting lost the order of the sets in flight
     // and waited for all of them before every store.
-    template <int S, bool ALL_FAST>
-    __device__ __forceinline__ static bool produce(Prod& t, int ptid, long long j, long long nkb, char* ring,
+    template <int S, int POL>
+    __device__ __forceinline__ static bool produce(Prod& t, int ptid, long long j, long long nloop, char* ring,
                                                    Shared& sh, const GemmParams& p, long long m0, long long n0,
                                                    long long kbeg, long long kend, bool full_mn, unsigned limit) {
         bool ok = true;
-        if (j < nkb) {
+        if (j < nloop) {
             const int slot = (int)(j % NS);
             const unsigned round = (unsigned)(j / NS);
             ok = round == 0 || wait_count(&sh.freed[slot], CWAVES * round, &sh.abort_flag, limit);
@@ -117,13 +135,42 @@ struct Feed {
                 signal(&sh.ready[slot]);
             }
         }
-        const long long jn = min(j + PFR, nkb - 1);
-        const long long k0 = kbeg + jn * BK;
-        t.template load_as<S>(ptid, p.A, p.B, p.M, p.N, p.lda, p.ldb, m0, n0, k0, kend,
-                              ALL_FAST || (full_mn && k0 + BK <= kend));
+        const long long jn = min(j + PFR, nloop - 1);
+        load_block<S, POL>(t, ptid, p, m0, n0, kbeg + jn * BK, kend, full_mn);
         return ok;
     }
 
+    // k-blocks [0, nloop) with PFR blocks of loads in flight, every block
+    // loaded under policy POL. False when a wait ran out.
+    template <int POL>
+    __device__ __forceinline__ static bool produce_blocks(Prod& t, int ptid, char* ring, Shared& sh,
+                                                          const GemmParams& p, long long m0, long long n0,
+                                                          long long kbeg, long long kend, bool full_mn,
+                                                          long long nloop, unsigned limit) {
+        auto fetch = [&](auto set, long long j) {
+            load_block<decltype(set)::value, POL>(t, ptid, p, m0, n0, kbeg + min(j, nloop - 1) * BK, kend, full_mn);
+        };
+        // Loads of the first PFR k-blocks, one register set each.
+        fetch(std::integral_constant<int, 0>{}, 0);
+        if constexpr (PFR > 1) fetch(std::integral_constant<int, 1>{}, 1);
+        if constexpr (PFR > 2) fetch(std::integral_constant<int, 2>{}, 2);
+        if constexpr (PFR > 3) fetch(std::integral_constant<int, 3>{}, 3);
+        for (long long j = 0; j < nloop; j += PFR) {
+            // Every step runs (its loads are unconditional); an aborted wait
+            // skips only that step's stores, and the loop ends after the group.
+            bool ok = produce<0, POL>(t, ptid, j, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
+            if constexpr (PFR > 1) ok &= produce<1, POL>(t, ptid, j + 1, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
+            if constexpr (PFR > 2) ok &= produce<2, POL>(t, ptid, j + 2, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
+            if constexpr (PFR > 3) ok &= produce<3, POL>(t, ptid, j + 3, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
+            if (!ok) return false;
+        }
+        return true;
+    }
+
+    // Full tiles with whole k-blocks: every block on the plain vector path.
+    // Otherwise, when N is a multiple of the B unit width, the k-blocks inside
+    // K run the clamped path and a last partial k-block is loaded and stored
+    // on its own afterwards; else every block takes the general path.
     template <bool ALL_FAST>
     __device__ __forceinline__ static void producer(char* ring, Shared& sh, const GemmParams& p, long long m0,
                                                     long long n0, long long kbeg, long long kend, bool full_mn,
@@ -131,24 +178,25 @@ struct Feed {
         if (nkb <= 0) return;
         Prod t;
         const int ptid = (int)threadIdx.x - CT;
-        auto fetch = [&](auto set, long long j) {
-            const long long k0 = kbeg + min(j, nkb - 1) * BK;
-            t.template load_as<decltype(set)::value>(ptid, p.A, p.B, p.M, p.N, p.lda, p.ldb, m0, n0, k0, kend,
-                                                     ALL_FAST || (full_mn && k0 + BK <= kend));
-        };
-        // Loads of the first PFR k-blocks, one register set each.
-        fetch(std::integral_constant<int, 0>{}, 0);
-        if constexpr (PFR > 1) fetch(std::integral_constant<int, 1>{}, 1);
-        if constexpr (PFR > 2) fetch(std::integral_constant<int, 2>{}, 2);
-        if constexpr (PFR > 3) fetch(std::integral_constant<int, 3>{}, 3);
-        for (long long j = 0; j < nkb; j += PFR) {
-            // Every step runs (its loads are unconditional); an aborted wait
-            // skips only that step's stores, and the loop ends after the group.
-            bool ok = produce<0, ALL_FAST>(t, ptid, j, nkb, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
-            if constexpr (PFR > 1) ok &= produce<1, ALL_FAST>(t, ptid, j + 1, nkb, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
-            if constexpr (PFR > 2) ok &= produce<2, ALL_FAST>(t, ptid, j + 2, nkb, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
-            if constexpr (PFR > 3) ok &= produce<3, ALL_FAST>(t, ptid, j + 3, nkb, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
-            if (!ok) return;
+        if constexpr (ALL_FAST) {
+            (void)produce_blocks<kLoadFull>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, limit);
+        } else {
+            const long long nwhole = (kend - kbeg) / BK;
+            if (p.N % Prod::BW == 0 && p.N >= Prod::BW && nwhole >= 1) {
+                if (!produce_blocks<kLoadEdge>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nwhole, limit))
+                    return;
+                if (nwhole < nkb) {  // the partial last k-block
+                    const long long j = nwhole;
+                    load_block<0, kLoadGeneral>(t, ptid, p, m0, n0, kbeg + j * BK, kend, full_mn);
+                    const int slot = (int)(j % NS);
+                    const unsigned round = (unsigned)(j / NS);
+                    if (round > 0 && !wait_count(&sh.freed[slot], CWAVES * round, &sh.abort_flag, limit)) return;
+                    t.template store_as<0>(ptid, ring + slot * STAGE);
+                    signal(&sh.ready[slot]);
+                }
+            } else {
+                (void)produce_blocks<kLoadGeneral>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, limit);
+            }
         }
     }
 
@@ -262,55 +310,59 @@ void set_feed_mode(int v) { g_feed.store(v, std::memory_order_relaxed); }
 
 // Variants (k-block depth BK, ring slots NS, k-blocks of loads in flight PFR)
 // and where the automatic setting takes them. Measured against the direct
-// tile on one MI355X: per-call times of chosen shapes, and every 64-tile
-// candidate of 1200 shapes of the selector's table (M, N, K log-uniform in
-// 1..8192, all three modes) timed by the autotuner under both feeds.
-//   3 split6 BK 64, NS 2, PFR 2: one workgroup per CU and K slices of at
-//     least 512: 1.06-1.40x on every such candidate with slices of 1024 or
-//     more (median 1.10 at 512-1023), e.g. 768 x 768 x 4096 (64,1) 76.3 ->
-//     63.0 us. Shorter slices lose (median 0.50 below 64, 0.93 at 128-255).
-//   1 split3 BK 128, NS 2, PFR 2, and 2 split3 BK 128, NS 2, PFR 1: split3
-//     gains only on grids of 64..256 workgroups with slices of at least 1024
-//     (1024^3 (64,1) 14.6 -> 13.0 us with 2, 768 x 768 x 4096 (64,1) 48.4 ->
-//     38.9 us with 1, 83 x 4819 x 6314 (64,1) 1.12x), and is even or 2-7%
-//     slower on smaller grids and shorter slices. Variant 1 (two blocks of
-//     loads in flight) for slices of 4096 or more, 2 below.
+// tile on one MI355X: per-call times of chosen shapes, and every 64- and
+// 128-tile candidate of 1200 shapes of the selector's table (M, N, K
+// log-uniform in 1..8192, all three modes; nearly all ragged) timed by the
+// autotuner under both feeds.
+//   3 split6 64 x 64, BK 64, NS 2, PFR 2: one workgroup per CU and K slices of
+//     at least 256: median 1.21x (p10 1.02x) over such candidates, 1.23x
+//     from 512 (768 x 768 x 4096 (64,1) 76.5 -> 63.4 us, 700 x 900 x 3000
+//     110.2 -> 55.9 us). Shorter slices lose (median 0.5-0.6x below 128).
+//   1 split3 64 x 64, BK 128, NS 2, PFR 2, and 2 split3 64 x 64, BK 128, NS 2,
+//     PFR 1: one workgroup per CU and slices of at least 1024: median 1.02x
+//     on the table's shapes, 1.05-1.33x on grids of 64..256 workgroups
+//     (1000^3 (64,1) 23.6 -> 17.5 us, 768 x 768 x 4096 48.3 -> 40.0 us).
+//     Variant 1 (two blocks of loads in flight) for slices of 4096 or more.
 //   7 split3 128 x 128, BK 64, NS 2, PFR 1, eight consumer and eight producer
-//     waves, and 8 split6 128 x 128, BK 32, NS 2, PFR 2, eight consumer and
-//     four producer waves: grids of at most one workgroup per CU whose tiles
-//     and k-blocks are all full (M, N multiples of 128; K and the slice
-//     multiples of 64), slices of at least 1024: 2048^3 (128,1) split3 61.7 ->
-//     59.7 us, 512 x 4096 x 4096 94.5 -> 83.7 us, 1152 x 1152 x 8192 184.2 ->
-//     161.6 us; split6 1.03-1.10x. Ragged tiles or a partial last k-block put
-//     the element-wise edge loads on the producers alone: 3-18% slower
-//     (1920 x 1920 x 2000, 1000 x 1100 x 3000), so those stay direct, as do
-//     grids of more than one workgroup per CU (4096 x 4096 x 1024: 0.94x).
-//     Plain keeps the direct 128 x 128 tile.
-//   4 plain BK 64, NS 2, PFR 2; 5 split3 BK 64, NS 2, PFR 2; 6 split6 BK 32,
-//     NS 4, PFR 2: only when the feed is forced. Plain was 3-40% slower on
-//     small grids with every variant, and on grids of more than one
+//     waves: one workgroup per CU and slices of at least 1024, when every
+//     tile and k-block is full (2048^3 (128,1) 63.8 -> 59.1 us, 512 x 4096 x
+//     4096 94.9 -> 84.4 us), or when M and N are at least 256, N is even and
+//     slices are at least 2048 (1000 x 1000 x 4000 136.4 -> 92.0 us;
+//     median 1.12x on the table's shapes). Smaller or thinner tiles lose up
+//     to 10%.
+//   8 split6 128 x 128, BK 32, NS 2, PFR 2, eight consumer and four producer
+//     waves: one workgroup per CU, every tile and k-block full, slices of at
+//     least 1024 (1152 x 1152 x 8192 (128,1) 315.7 -> 288.9 us); on the
+//     table's ragged shapes it lost 10-25%.
+//   4 plain 64 x 64, BK 64, NS 2, PFR 2; 5 split3 BK 64, NS 2, PFR 2; 6 split6
+//     BK 32, NS 4, PFR 2: only when the feed is forced. Plain was 10-40%
+//     slower on small grids with every variant, and on grids of more than one
 //     workgroup per CU all modes were 1-25% slower (the direct tile hides
 //     latency with several workgroups per CU instead).
-// Slices shorter than the k-block depth are zero-padded, as in the deep
-// direct variants: results are those of the direct tile bit for bit, except
-// that an exact -0 may come out as +0.
+// Producers load a k-block inside K on a tile that crosses M or N with
+// clamped vector loads (load_edge_as) when N is a multiple of their B unit
+// width, and a partial last k-block on its own after the others; the direct
+// tile keeps element-wise edge loads. Slices shorter than the k-block depth
+// are zero-padded, as in the deep direct variants: results are those of the
+// direct tile bit for bit, except that an exact -0 may come out as +0.
 int feed_variant(int mode, int cfg, long long M, long long N, long long K, long long grid, long long k_len,
                  int setting) {
     if (setting == 0 || mode < 0 || mode > 2) return 0;
     const bool small_grid = grid > 0 && grid <= kDeepMaxGrid;
     if (cfg == 3) {
-        // 128 x 128 (split modes): only where every tile and k-block is full;
-        // plain keeps the direct tile.
+        // 128 x 128 (split modes); plain keeps the direct tile.
         if (mode == 0) return 0;
         if (setting == 1) return mode == 1 ? 7 : 8;
         const bool whole = M % 128 == 0 && N % 128 == 0 && K % 64 == 0 && k_len % 64 == 0;
-        return (small_grid && whole && k_len >= 1024) ? (mode == 1 ? 7 : 8) : 0;
+        if (!small_grid || k_len < 1024) return 0;
+        if (mode == 1) return (whole || (M >= 256 && N >= 256 && N % 2 == 0 && k_len >= 2048)) ? 7 : 0;
+        return whole ? 8 : 0;
     }
     if (cfg != 0) return 0;
     if (setting == 2) {  // measured rule
         if (!small_grid) return 0;
-        if (mode == 1) return (grid >= 64 && k_len >= 1024) ? (k_len >= 4096 ? 1 : 2) : 0;
-        return (mode == 2 && k_len >= 512) ? 3 : 0;
+        if (mode == 1) return k_len >= 1024 ? (k_len >= 4096 ? 1 : 2) : 0;
+        return (mode == 2 && k_len >= 256) ? 3 : 0;
     }
     if (mode == 0) return 4;
     if (mode == 1) return small_grid ? (k_len >= 4096 ? 1 : 2) : 5;

@@ -198,6 +198,49 @@ struct Tile {
         }
     }
 
+    // load_as() for a k-block wholly inside the K range on a tile that may
+    // cross M or N, when N is a multiple of BW (a B unit is then wholly inside
+    // or wholly outside N). Every unit is one unconditional vector load: units
+    // past M or N read a clamped in-range address and are zeroed by select.
+    // Without branches around the loads, a caller that keeps several blocks of
+    // loads in flight keeps them in flight (the compiler's wait counting loses
+    // their order across conditional loads).
+    template <int S>
+    __device__ __forceinline__ void load_edge_as(int tid, const float* __restrict__ A, const float* __restrict__ B,
+                                                 long long M, long long N, long long lda, long long ldb,
+                                                 long long m0, long long n0, long long k0) {
+#pragma unroll
+        for (int i = 0; i < UA; ++i) {
+            const int u = tid + NT * i;
+            if (A_UNITS % NT == 0 || u < A_UNITS) {
+                const int row = u / CPR, c = u % CPR;
+                const long long gm = m0 + row, gk = k0 + c * LU;
+                const bool in = gm < M;
+                const float* src = A + (in ? gm : M - 1) * lda + gk;
+#pragma unroll
+                for (int q = 0; q < LU / 4; ++q) {
+                    const f32x4 v = *reinterpret_cast<const f32x4u*>(src + 4 * q);
+                    ra[S][i][q] = in ? v : f32x4{0.f, 0.f, 0.f, 0.f};
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < UB; ++i) {
+            const int u = tid + NT * i;
+            if (B_UNITS % NT == 0 || u < B_UNITS) {
+                const int nb = u % (BN / BW), c = u / (BN / BW);
+                const long long gn = n0 + BW * nb, gk = k0 + c * LU;
+                const bool in = gn < N;
+                const float* src = B + gk * ldb + (in ? gn : N - BW);
+#pragma unroll
+                for (int e = 0; e < LU; ++e) {
+                    const bvec v = *reinterpret_cast<const bvecu*>(src + e * ldb);
+                    rb[S][i][e] = in ? v : bvec(0.f);
+                }
+            }
+        }
+    }
+
     template <int S>
     __device__ __forceinline__ void store(char* stage) const { store_as<S>(threadIdx.x, stage); }
 
@@ -368,6 +411,11 @@ struct Tile {
     // before they are needed. Two sets (PF = 2): the sets alternate and the
     // loop is unrolled by two, so blocks kb+2 and kb+3 are both in flight
     // while block kb is multiplied.
+    //
+    // (The feed's clamped edge loads, load_edge_as, measured slower here:
+    // on tiles inside M and N they only add selects, and on tiles mostly
+    // past M they re-read the clamped row where the element-wise path loads
+    // nothing; the 256-problem grouped batch went from 12.7 to 14.2 ms.)
     template <bool ALL_FAST>
     __device__ __forceinline__ void mainloop(const float* __restrict__ A, const float* __restrict__ B,
                                              long long M, long long N, long long lda, long long ldb,

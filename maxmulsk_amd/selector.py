@@ -117,7 +117,7 @@ class Trainer:
                                             self.v.data_ptr(), self.t, lr, 0.9, 0.999, 1e-8, None))
 
 
-def train(X: np.ndarray, T: np.ndarray, Mk: np.ndarray, steps: int = 20000, batch: int = 1024,
+def train(X: np.ndarray, T: np.ndarray, Mk: np.ndarray, steps: int = 40000, batch: int = 1024,
           lr: float = 3e-3, seed: int = 7, log=print) -> tuple[np.ndarray, list[float]]:
     import torch
     rng = np.random.default_rng(seed)
@@ -181,7 +181,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=5000)
     ap.add_argument("--eval", type=int, default=500)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=40000)
     ap.add_argument("--out", default=os.path.dirname(os.path.abspath(__file__)))
     ap.add_argument("--dataset", default="", help="reuse a collected dataset instead of timing")
     ap.add_argument("--stage1", type=int, default=5, help="autotuner repetitions per candidate")

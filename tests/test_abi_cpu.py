@@ -399,12 +399,13 @@ def test_lds_patterns_of_the_decoupled_feed(mode, rows, bk, threads, b_cols):
 
 def test_feed_variant_rule():
     """Which kernel a launch takes under each feed setting (pure host code):
-    the automatic rule feeds 64 x 64 split6 launches of at most one workgroup
-    per CU with K slices of at least 512, 64 x 64 split3 launches of 64..256
-    workgroups with slices of at least 1024, and 128 x 128 split-mode launches
-    of at most one workgroup per CU whose tiles and k-blocks are all full;
-    forcing the feed picks a variant for every 64 x 64 launch and every
-    split-mode 128 x 128 launch; other geometries never take it."""
+    the automatic rule feeds launches of at most one workgroup per CU: 64 x 64
+    split6 with K slices of at least 256, 64 x 64 split3 with slices of at
+    least 1024, 128 x 128 split3 and split6 whose tiles and k-blocks are all
+    full (slices of at least 1024), and 128 x 128 split3 with M, N >= 256,
+    even N and slices of at least 2048; forcing the feed picks a variant for
+    every 64 x 64 launch and every split-mode 128 x 128 launch; other
+    geometries never take it."""
     prev = mm.get_feed()
     try:
         mm.set_feed(mm.FEED_AUTO)
@@ -413,15 +414,20 @@ def test_feed_variant_rule():
             (768, 768, 4096, (64, 1)): (0, 1, 3),
             (640, 640, 3000, (64, 1)): (0, 2, 3),
             (512, 768, 4500, (64, 1)): (0, 1, 3),
-            (512, 384, 2500, (64, 1)): (0, 0, 3),
-            (64, 512, 8192, (64, 32)): (0, 0, 0),
-            (64, 512, 8192, (64, 16)): (0, 0, 3),
-            (67, 513, 8191, (64, 16)): (0, 0, 0),
+            (512, 384, 2500, (64, 1)): (0, 2, 3),           # 48 workgroups
+            (64, 512, 8192, (64, 32)): (0, 0, 3),           # slices of 256
+            (64, 512, 8192, (64, 64)): (-1, -1, -1),        # split above 32
+            (64, 512, 4096, (64, 32)): (0, 0, 0),           # slices of 128
+            (64, 512, 8192, (64, 16)): (0, 0, 3),           # split3: slices of 512
+            (67, 513, 8191, (64, 16)): (0, 0, 0),           # 288 workgroups
             (4096, 4096, 4096, (256, 1)): (0, 0, 0),
             (1024, 1024, 1024, (128, 1)): (0, 7, 8),        # 128 x 128, full tiles and k-blocks
             (2048, 2048, 2048, (128, 1)): (0, 7, 8),
-            (1920, 1920, 2000, (128, 1)): (0, 0, 0),        # K not a multiple of 64
-            (1000, 1100, 3000, (128, 1)): (0, 0, 0),        # ragged tiles
+            (1920, 1920, 2000, (128, 1)): (0, 0, 0),        # partial k-block, slices < 2048
+            (1000, 1000, 4000, (128, 1)): (0, 7, 0),        # ragged, slices >= 2048
+            (1000, 1100, 3000, (128, 1)): (0, 7, 0),
+            (1000, 1101, 3000, (128, 1)): (0, 0, 0),        # odd N
+            (200, 1100, 3000, (128, 1)): (0, 0, 0),         # M < 256
             (2048, 2048, 512, (128, 1)): (0, 0, 0),         # slices of 512
             (4096, 4096, 1024, (128, 1)): (0, 0, 0),        # 1024 workgroups
         }

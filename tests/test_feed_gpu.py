@@ -118,15 +118,15 @@ AUTO = [
     ((1024, 1024, 1024, (64, 1)), {0: 0, 1: 2, 2: 3}),
     ((768, 768, 4096, (64, 1)), {0: 0, 1: 1, 2: 3}),
     ((512, 512, 8192, (64, 4)), {0: 0, 1: 2, 2: 3}),
-    ((64, 512, 8192, (64, 32)), {0: 0, 1: 0, 2: 0}),      # slices of 256: direct
+    ((64, 512, 8192, (64, 32)), {0: 0, 1: 0, 2: 3}),      # split3: slices of 256, direct
     ((64, 512, 8192, (64, 16)), {0: 0, 1: 0, 2: 3}),      # split3: slices of 512
-    ((512, 384, 2500, (64, 1)), {0: 0, 1: 0, 2: 3}),      # split3: 48 workgroups
+    ((512, 384, 2500, (64, 1)), {0: 0, 1: 2, 2: 3}),
     ((67, 513, 8191, (64, 16)), {0: 0, 1: 0, 2: 0}),      # 288 workgroups: direct
     ((64, 7168, 16384, (64, 8)), {0: 0, 1: 0, 2: 0}),
     ((4096, 4096, 4096, (256, 1)), {0: 0, 1: 0, 2: 0}),   # packed path
     ((1024, 1024, 1024, (128, 1)), {0: 0, 1: 7, 2: 8}),   # 128 x 128, full tiles and k-blocks
-    ((1920, 1920, 2000, (128, 1)), {0: 0, 1: 0, 2: 0}),   # partial last k-block
-    ((1000, 1100, 3000, (128, 1)), {0: 0, 1: 0, 2: 0}),   # ragged tiles
+    ((1920, 1920, 2000, (128, 1)), {0: 0, 1: 0, 2: 0}),   # partial last k-block, slices < 2048
+    ((1000, 1100, 3000, (128, 1)), {0: 0, 1: 7, 2: 0}),   # ragged: split3 from slices of 2048
 ]
 
 
@@ -138,7 +138,8 @@ def test_auto_rule_and_results(feed_setting):
     # an automatically fed launch gives the direct tile's C, inside the FP32 bound
     for mode, (M, N, K, plan) in ((1, (640, 640, 3000, (64, 1))), (1, (512, 768, 4500, (64, 1))),
                                   (2, (200, 300, 1200, (64, 2))), (2, (128, 1024, 4096, (64, 8))),
-                                  (1, (512, 384, 2048, (128, 1))), (2, (384, 512, 1024, (128, 1)))):
+                                  (1, (512, 384, 2048, (128, 1))), (2, (384, 512, 1024, (128, 1))),
+                                  (1, (300, 500, 2500, (128, 1))), (2, (130, 70, 300, (64, 1)))):
         A = orc.uniform(21, (M, K))
         B = orc.uniform(22, (K, N))
         mm.set_feed(mm.FEED_AUTO)
