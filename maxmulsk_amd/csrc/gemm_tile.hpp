@@ -412,10 +412,12 @@ struct Tile {
     // loop is unrolled by two, so blocks kb+2 and kb+3 are both in flight
     // while block kb is multiplied.
     //
-    // (The feed's clamped edge loads, load_edge_as, measured slower here:
-    // on tiles inside M and N they only add selects, and on tiles mostly
-    // past M they re-read the clamped row where the element-wise path loads
-    // nothing; the 256-problem grouped batch went from 12.7 to 14.2 ms.)
+    // (Measured slower here and not used: the feed's clamped edge loads,
+    // load_edge_as, which add selects on tiles inside M and N and re-read the
+    // clamped row on tiles mostly past M (the 256-problem grouped batch went
+    // from 12.7 to 14.2 ms); and choosing per k-block between a straight run
+    // of vector loads and the general path (split6 128 x 128 on 2000^3 went
+    // from 109 to 127 us).)
     template <bool ALL_FAST>
     __device__ __forceinline__ void mainloop(const float* __restrict__ A, const float* __restrict__ B,
                                              long long M, long long N, long long lda, long long ldb,
