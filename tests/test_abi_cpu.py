@@ -448,3 +448,22 @@ def test_feed_variant_rule():
         assert mm.get_feed() == mm.FEED_DIRECT
     finally:
         mm.set_feed(prev)
+
+
+def test_stream_probe_rejects_bad_arguments_before_any_device_call():
+    """maxmulsk_probe_stream refuses the packed plan, an invalid plan, K or N not
+    a multiple of 4 and a non-positive call count with MAXMULSK_ERR_UNSUPPORTED,
+    before it touches a device; without a device a valid request is refused
+    the same way instead of crashing."""
+    import ctypes as C
+    lib = mm.lib()
+    us = (C.c_double * 3)()
+    bad = [(256, 256, 256, (256, 1), 10), (64, 512, 512, (48, 1), 10), (64, 255, 512, (64, 1), 10),
+           (64, 512, 255, (64, 1), 10), (64, 512, 512, (64, 1), 0), (64, 512, 512, (64, 33), 10)]
+    for M, K, N, plan, calls in bad:
+        st = lib.maxmulsk_probe_stream(M, K, N, C.byref(mm.Plan(*plan)), calls, None, us)
+        assert st == mm.ERR_UNSUPPORTED, (M, K, N, plan, calls)
+    if mm.device_count() == 0:
+        st = lib.maxmulsk_probe_stream(64, 512, 512, C.byref(mm.Plan(64, 1)), 10, None, us)
+        assert st == mm.ERR_UNSUPPORTED
+        assert "device" in mm.last_error()
