@@ -323,7 +323,7 @@ maxmulsk_status maxmulsk_probe_mfma(int mode, int random_operands, int iteration
  * slices of at least 256, 64 x 64 split3 with slices of at least 1024,
  * 128 x 128 split3/split6 whose tiles and k-blocks are all full (M, N
  * multiples of 128, K and slices multiples of 64, slices of at least 1024),
- * and 128 x 128 split3 with M, N >= 256, even N and slices of at least 2048;
+ * and 128 x 128 split3 with M, N >= 256 and slices of at least 2048;
  * the direct tile elsewhere. Both feeds give the same C (same K
  * slices, products and order), except that an exact -0 may come out as +0.
  * Grouped launches keep their own kernel. Counterpart of the reference's Trio

@@ -98,9 +98,9 @@ struct Feed {
     };
 
     // How a producer step loads a k-block: every unit a plain vector load
-    // (full tiles), one clamped vector load per unit (k-block inside K, tile
-    // past M or N, N a multiple of the B unit width), or the tile's general
-    // path (vector loads inside, element-wise across an edge).
+    // (k-block inside M, N and K), one clamped vector load per unit (k-block
+    // inside K, tile across M or N), or the tile's general path (vector loads
+    // inside, element-wise across an edge).
     enum { kLoadFull = 0, kLoadEdge = 1, kLoadGeneral = 2 };
 
     template <int S, int POL>
@@ -168,9 +168,11 @@ struct Feed {
     }
 
     // Full tiles with whole k-blocks: every block on the plain vector path.
-    // Otherwise, when N is a multiple of the B unit width, the k-blocks inside
-    // K run the clamped path and a last partial k-block is loaded and stored
-    // on its own afterwards; else every block takes the general path.
+    // Otherwise the k-blocks inside K run the plain vector path (tile inside
+    // M and N) or the clamped one (tile across M or N, N at least the B unit
+    // width), and a last partial k-block is loaded and stored on its own
+    // afterwards; with N below the unit width every block takes the general
+    // path.
     template <bool ALL_FAST>
     __device__ __forceinline__ static void producer(char* ring, Shared& sh, const GemmParams& p, long long m0,
                                                     long long n0, long long kbeg, long long kend, bool full_mn,
@@ -182,9 +184,11 @@ struct Feed {
             (void)produce_blocks<kLoadFull>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, limit);
         } else {
             const long long nwhole = (kend - kbeg) / BK;
-            if (p.N % Prod::BW == 0 && p.N >= Prod::BW && nwhole >= 1) {
-                if (!produce_blocks<kLoadEdge>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nwhole, limit))
-                    return;
+            if (p.N >= Prod::BW && nwhole >= 1) {
+                const bool whole_ok =
+                    full_mn ? produce_blocks<kLoadFull>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nwhole, limit)
+                            : produce_blocks<kLoadEdge>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nwhole, limit);
+                if (!whole_ok) return;
                 if (nwhole < nkb) {  // the partial last k-block
                     const long long j = nwhole;
                     load_block<0, kLoadGeneral>(t, ptid, p, m0, n0, kbeg + j * BK, kend, full_mn);
@@ -326,8 +330,8 @@ void set_feed_mode(int v) { g_feed.store(v, std::memory_order_relaxed); }
 //   7 split3 128 x 128, BK 64, NS 2, PFR 1, eight consumer and eight producer
 //     waves: one workgroup per CU and slices of at least 1024, when every
 //     tile and k-block is full (2048^3 (128,1) 63.8 -> 59.1 us, 512 x 4096 x
-//     4096 94.9 -> 84.4 us), or when M and N are at least 256, N is even and
-//     slices are at least 2048 (1000 x 1000 x 4000 136.4 -> 92.0 us;
+//     4096 94.9 -> 84.4 us), or when M and N are at least 256 and slices are
+//     at least 2048 (1000 x 1000 x 4000 136.4 -> 92.0 us;
 //     median 1.12x on the table's shapes). Smaller or thinner tiles lose up
 //     to 10%.
 //   8 split6 128 x 128, BK 32, NS 2, PFR 2, eight consumer and four producer
@@ -340,9 +344,8 @@ void set_feed_mode(int v) { g_feed.store(v, std::memory_order_relaxed); }
 //     workgroup per CU all modes were 1-25% slower (the direct tile hides
 //     latency with several workgroups per CU instead).
 // Producers load a k-block inside K on a tile that crosses M or N with
-// clamped vector loads (load_edge_as) when N is a multiple of their B unit
-// width, and a partial last k-block on its own after the others; the direct
-// tile keeps element-wise edge loads. Slices shorter than the k-block depth
+// clamped vector loads (load_edge_as), and a partial last k-block on its own
+// after the others; the direct tile keeps element-wise edge loads. Slices shorter than the k-block depth
 // are zero-padded, as in the deep direct variants: results are those of the
 // direct tile bit for bit, except that an exact -0 may come out as +0.
 int feed_variant(int mode, int cfg, long long M, long long N, long long K, long long grid, long long k_len,
@@ -355,7 +358,7 @@ int feed_variant(int mode, int cfg, long long M, long long N, long long K, long 
         if (setting == 1) return mode == 1 ? 7 : 8;
         const bool whole = M % 128 == 0 && N % 128 == 0 && K % 64 == 0 && k_len % 64 == 0;
         if (!small_grid || k_len < 1024) return 0;
-        if (mode == 1) return (whole || (M >= 256 && N >= 256 && N % 2 == 0 && k_len >= 2048)) ? 7 : 0;
+        if (mode == 1) return (whole || (M >= 256 && N >= 256 && k_len >= 2048)) ? 7 : 0;
         return whole ? 8 : 0;
     }
     if (cfg != 0) return 0;

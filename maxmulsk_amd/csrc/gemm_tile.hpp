@@ -199,12 +199,13 @@ struct Tile {
     }
 
     // load_as() for a k-block wholly inside the K range on a tile that may
-    // cross M or N, when N is a multiple of BW (a B unit is then wholly inside
-    // or wholly outside N). Every unit is one unconditional vector load: units
-    // past M or N read a clamped in-range address and are zeroed by select.
-    // Without branches around the loads, a caller that keeps several blocks of
-    // loads in flight keeps them in flight (the compiler's wait counting loses
-    // their order across conditional loads).
+    // cross M or N (N at least BW). Every unit is one unconditional vector
+    // load: A units past M read the clamped last row and are zeroed by select;
+    // a B unit that crosses N reads the last BW columns and shifts them into
+    // place by selects, and one past N reads them and is zeroed. Without
+    // branches around the loads, a caller that keeps several blocks of loads in
+    // flight keeps them in flight (the compiler's wait counting loses their
+    // order across conditional loads).
     template <int S>
     __device__ __forceinline__ void load_edge_as(int tid, const float* __restrict__ A, const float* __restrict__ B,
                                                  long long M, long long N, long long lda, long long ldb,
@@ -230,12 +231,21 @@ struct Tile {
             if (B_UNITS % NT == 0 || u < B_UNITS) {
                 const int nb = u % (BN / BW), c = u / (BN / BW);
                 const long long gn = n0 + BW * nb, gk = k0 + c * LU;
-                const bool in = gn < N;
-                const float* src = B + gk * ldb + (in ? gn : N - BW);
+                const long long base = gn + BW <= N ? gn : N - BW;  // in range: N >= BW
+                const int sh = gn < N ? (int)(gn - base) : 0;       // 0 inside, 1..BW-1 across N
+                const float* src = B + gk * ldb + base;
 #pragma unroll
                 for (int e = 0; e < LU; ++e) {
                     const bvec v = *reinterpret_cast<const bvecu*>(src + e * ldb);
-                    rb[S][i][e] = in ? v : bvec(0.f);
+                    bvec o;
+#pragma unroll
+                    for (int j = 0; j < BW; ++j) {
+                        float x = v[j];
+#pragma unroll
+                        for (int t = 1; t < BW - j; ++t) x = sh == t ? v[j + t] : x;
+                        o[j] = gn + j < N ? x : 0.f;
+                    }
+                    rb[S][i][e] = o;
                 }
             }
         }

@@ -402,8 +402,8 @@ def test_feed_variant_rule():
     the automatic rule feeds launches of at most one workgroup per CU: 64 x 64
     split6 with K slices of at least 256, 64 x 64 split3 with slices of at
     least 1024, 128 x 128 split3 and split6 whose tiles and k-blocks are all
-    full (slices of at least 1024), and 128 x 128 split3 with M, N >= 256,
-    even N and slices of at least 2048; forcing the feed picks a variant for
+    full (slices of at least 1024), and 128 x 128 split3 with M, N >= 256
+    and slices of at least 2048; forcing the feed picks a variant for
     every 64 x 64 launch and every split-mode 128 x 128 launch; other
     geometries never take it."""
     prev = mm.get_feed()
@@ -426,7 +426,7 @@ def test_feed_variant_rule():
             (1920, 1920, 2000, (128, 1)): (0, 0, 0),        # partial k-block, slices < 2048
             (1000, 1000, 4000, (128, 1)): (0, 7, 0),        # ragged, slices >= 2048
             (1000, 1100, 3000, (128, 1)): (0, 7, 0),
-            (1000, 1101, 3000, (128, 1)): (0, 0, 0),        # odd N
+            (1000, 1101, 3000, (128, 1)): (0, 7, 0),        # odd N
             (200, 1100, 3000, (128, 1)): (0, 0, 0),         # M < 256
             (2048, 2048, 512, (128, 1)): (0, 0, 0),         # slices of 512
             (4096, 4096, 1024, (128, 1)): (0, 0, 0),        # 1024 workgroups
