@@ -51,7 +51,38 @@ __global__ __launch_bounds__(256) void pack_kernel(PackJob a, PackJob b, long lo
     const long long r0 = (long long)rb * 32;
     const long long k0 = (long long)blockIdx.y * kPackKT;
     const int tid = threadIdx.x;
-    if (!k_major) {
+    // A super-tile wholly inside the operand and contiguous along its read
+    // direction issues all eight loads before its first LDS write, so they
+    // are in flight together; the guarded loads below cannot be, because the
+    // compiler waits for each one inside its branch.
+    const bool whole = r0 + 32 <= R && k0 + kPackKT <= K && (k_major ? ld_row == 1 : ld_k == 1);
+    if (whole) {
+        f32x4 v[8];
+        if (!k_major) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int idx = tid + 256 * i;
+                v[i] = *reinterpret_cast<const f32x4u*>(X + (r0 + (idx >> 6)) * ld_row + k0 + 4 * (idx & 63));
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int idx = tid + 256 * i;
+                *reinterpret_cast<f32x4*>(&tile[(idx >> 6) * kPackPitch + 4 * (idx & 63)]) = v[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int idx = tid + 256 * i;
+                v[i] = *reinterpret_cast<const f32x4u*>(X + (k0 + (idx >> 3)) * ld_k + r0 + 4 * (idx & 7));
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int idx = tid + 256 * i;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) tile[(4 * (idx & 7) + e) * kPackPitch + (idx >> 3)] = v[i][e];
+            }
+        }
+    } else if (!k_major) {
         // rows of A: 256 contiguous k per row, 64 float4 per row
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
