@@ -502,7 +502,18 @@ def bench_grouped(torch, mm, inputs, args, log, rounds) -> dict:
         gps[mode_name] = mm.GroupedPlan(probs, mode=mode)
         log(f"grouped {mode_name}: autotuned {args.grouped_classes} classes in {tune_s:.1f} s; "
             f"{plans_by_mode[mode_name]['problems_plan_changed_by_autotuner']} problems changed plan")
-    times = rounds({n: gp.launch for n, gp in gps.items()}, 5)
+    # The split modes' launches run on the decoupled feed by default; the direct
+    # grouped kernel on the same plans is timed beside them for context.
+    directs = {}
+    prev = mm.set_feed(mm.FEED_DIRECT)
+    for mode_name, mode in (("plain", 0), ("split3", 1)):
+        if gps[mode_name].info()["feed"]:
+            directs[mode_name] = mm.GroupedPlan(probs, mode=mode)
+    mm.set_feed(prev)
+    times = rounds({**{n: gp.launch for n, gp in gps.items()},
+                    **{n + "_direct": gp.launch for n, gp in directs.items()}}, 5)
+    for gp in directs.values():
+        gp.close()
     for mode_name, gp in gps.items():
         info = gp.info()
         # every problem checked against FP64 after a replay
@@ -524,11 +535,16 @@ def bench_grouped(torch, mm, inputs, args, log, rounds) -> dict:
         ms = float(np.median(times[mode_name]))
         res[mode_name] = {"ms": ms, "spread_pct": spread_pct(times[mode_name]),
                           "tflops": flops / (ms * 1e-3) / 1e12, "work_items": info["work_items"],
-                          "graph": info["graph"], "plans": plans_by_mode[mode_name],
+                          "graph": info["graph"], "decoupled_feed": info["feed"],
+                          "direct_kernel_ms": (float(np.median(times[mode_name + "_direct"]))
+                                               if mode_name + "_direct" in times else None),
+                          "plans": plans_by_mode[mode_name],
                           "errors_worst_of_256": worst,
                           "within_fp32_bound": worst["bound_ratio"] <= 1.0 and worst["nonfinite_problems"] == 0}
+        dk = res[mode_name]["direct_kernel_ms"]
         log(f"grouped 256 problems {mode_name}: {ms:.3f} ms {res[mode_name]['tflops']:.1f} TFLOP/s "
-            f"items {info['work_items']} worst bound ratio {worst['bound_ratio']:.3g}")
+            f"items {info['work_items']} worst bound ratio {worst['bound_ratio']:.3g}"
+            + (f"; decoupled feed (direct grouped kernel {dk:.3f} ms)" if dk else ""))
     return res
 
 

@@ -192,6 +192,12 @@ maxmulsk_status maxmulsk_grouped_launch(maxmulsk_grouped_plan plan, void* stream
  * FLOPs (2MNK summed over problems) and whether the launch replays a graph. */
 maxmulsk_status maxmulsk_grouped_info(maxmulsk_grouped_plan plan, size_t* work_items,
                                       double* flops, int* used_graph);
+/* 1 when the plan's launch runs on the decoupled feed (the persistent kernel
+ * with producer and consumer waves per workgroup, as in the reference's Trio,
+ * sme/trio/trio_sme.cpp:274-334), else 0. Split-mode plans take it unless they
+ * were built under MAXMULSK_FEED_DIRECT; the work items, tiles and K slices are
+ * those of the direct kernel, and so is the result, bit for bit. */
+int maxmulsk_grouped_uses_feed(maxmulsk_grouped_plan plan);
 void maxmulsk_grouped_destroy(maxmulsk_grouped_plan plan);
 
 /* Plan a problem gets inside a grouped launch when none is given: the grouped
@@ -326,7 +332,10 @@ maxmulsk_status maxmulsk_probe_mfma(int mode, int random_operands, int iteration
  * and 128 x 128 split3 with M, N >= 256 and slices of at least 2048;
  * the direct tile elsewhere. Both feeds give the same C (same K
  * slices, products and order), except that an exact -0 may come out as +0.
- * Grouped launches keep their own kernel. Counterpart of the reference's Trio
+ * Split-mode grouped launches run on the feed under DECOUPLED and AUTO (it
+ * measured 1.01-1.25x faster on every batch tried), on the direct grouped
+ * kernel under DIRECT; the setting is read when the grouped plan is created.
+ * Plain grouped launches keep their own kernel. Counterpart of the reference's Trio
  * engine, one compute thread fed by separate packing threads through rings
  * (sme/trio/trio.hpp:10-24, sme/trio/trio_sme.cpp:274-334). The setting is
  * process-wide and read at each launch; the first use reads

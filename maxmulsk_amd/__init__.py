@@ -84,6 +84,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_grouped_create": (C.c_int, [C.POINTER(Gemm), _SIZE, C.c_int, C.POINTER(Plan), _P, C.POINTER(_P)]),
     "maxmulsk_grouped_launch": (C.c_int, [_P, _P]),
     "maxmulsk_grouped_info": (C.c_int, [_P, C.POINTER(_SIZE), C.POINTER(C.c_double), C.POINTER(C.c_int)]),
+    "maxmulsk_grouped_uses_feed": (C.c_int, [_P]),
     "maxmulsk_grouped_destroy": (None, [_P]),
     "maxmulsk_grouped_default_plan": (Plan, [_SIZE, _SIZE, _SIZE, C.c_int]),
     "maxmulsk_grouped_plans": (C.c_int, [C.POINTER(Gemm), _SIZE, C.c_int, C.POINTER(Plan)]),
@@ -344,7 +345,8 @@ class GroupedPlan:
         flops = C.c_double()
         graph = C.c_int()
         check(lib().maxmulsk_grouped_info(self.handle, C.byref(items), C.byref(flops), C.byref(graph)))
-        return {"work_items": items.value, "flops": flops.value, "graph": bool(graph.value)}
+        return {"work_items": items.value, "flops": flops.value, "graph": bool(graph.value),
+                "feed": bool(lib().maxmulsk_grouped_uses_feed(self.handle))}
 
     def close(self) -> None:
         if self.handle:

@@ -75,6 +75,14 @@ __device__ __forceinline__ void signal(unsigned* cnt) {
     if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Ring header: per-slot counters and the workgroup abort flag.
+template <int NS>
+struct FeedShared {
+    unsigned ready[NS];
+    unsigned freed[NS];
+    unsigned abort_flag;
+};
+
 // A BM x BN tile with a CWM x CWN wave layout of consumers (the direct tile's
 // MFMA layout) and a PWM x PWN layout of producers (the direct tile's staging
 // map for that many threads); both sides address the same fragment-major slot
@@ -91,11 +99,8 @@ struct Feed {
     static_assert(CT + PT <= 1024, "one workgroup");
     static_assert(RING <= 160 * 1024 - 256, "LDS budget of one CU");
 
-    struct Shared {
-        unsigned ready[NS];
-        unsigned freed[NS];
-        unsigned abort_flag;
-    };
+    static constexpr int kBM = BM, kBN = BN, kBK = BK, kNS = NS;
+    using Shared = FeedShared<NS>;
 
     // How a producer step loads a k-block: every unit a plain vector load
     // (k-block inside M, N and K), one clamped vector load per unit (k-block
@@ -124,11 +129,12 @@ struct Feed {
     template <int S, int POL>
     __device__ __forceinline__ static bool produce(Prod& t, int ptid, long long j, long long nloop, char* ring,
                                                    Shared& sh, const GemmParams& p, long long m0, long long n0,
-                                                   long long kbeg, long long kend, bool full_mn, unsigned limit) {
+                                                   long long kbeg, long long kend, bool full_mn, long long jb,
+                                                   unsigned limit) {
         bool ok = true;
         if (j < nloop) {
-            const int slot = (int)(j % NS);
-            const unsigned round = (unsigned)(j / NS);
+            const int slot = (int)((jb + j) % NS);
+            const unsigned round = (unsigned)((jb + j) / NS);
             ok = round == 0 || wait_count(&sh.freed[slot], CWAVES * round, &sh.abort_flag, limit);
             if (ok) {
                 t.template store_as<S>(ptid, ring + slot * STAGE);
@@ -146,7 +152,7 @@ struct Feed {
     __device__ __forceinline__ static bool produce_blocks(Prod& t, int ptid, char* ring, Shared& sh,
                                                           const GemmParams& p, long long m0, long long n0,
                                                           long long kbeg, long long kend, bool full_mn,
-                                                          long long nloop, unsigned limit) {
+                                                          long long nloop, long long jb, unsigned limit) {
         auto fetch = [&](auto set, long long j) {
             load_block<decltype(set)::value, POL>(t, ptid, p, m0, n0, kbeg + min(j, nloop - 1) * BK, kend, full_mn);
         };
@@ -158,10 +164,10 @@ struct Feed {
         for (long long j = 0; j < nloop; j += PFR) {
             // Every step runs (its loads are unconditional); an aborted wait
             // skips only that step's stores, and the loop ends after the group.
-            bool ok = produce<0, POL>(t, ptid, j, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
-            if constexpr (PFR > 1) ok &= produce<1, POL>(t, ptid, j + 1, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
-            if constexpr (PFR > 2) ok &= produce<2, POL>(t, ptid, j + 2, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
-            if constexpr (PFR > 3) ok &= produce<3, POL>(t, ptid, j + 3, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, limit);
+            bool ok = produce<0, POL>(t, ptid, j, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, jb, limit);
+            if constexpr (PFR > 1) ok &= produce<1, POL>(t, ptid, j + 1, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, jb, limit);
+            if constexpr (PFR > 2) ok &= produce<2, POL>(t, ptid, j + 2, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, jb, limit);
+            if constexpr (PFR > 3) ok &= produce<3, POL>(t, ptid, j + 3, nloop, ring, sh, p, m0, n0, kbeg, kend, full_mn, jb, limit);
             if (!ok) return false;
         }
         return true;
@@ -176,36 +182,37 @@ struct Feed {
     template <bool ALL_FAST>
     __device__ __forceinline__ static void producer(char* ring, Shared& sh, const GemmParams& p, long long m0,
                                                     long long n0, long long kbeg, long long kend, bool full_mn,
-                                                    long long nkb, unsigned limit) {
+                                                    long long nkb, long long jb, unsigned limit) {
         if (nkb <= 0) return;
         Prod t;
         const int ptid = (int)threadIdx.x - CT;
         if constexpr (ALL_FAST) {
-            (void)produce_blocks<kLoadFull>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, limit);
+            (void)produce_blocks<kLoadFull>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, jb, limit);
         } else {
             const long long nwhole = (kend - kbeg) / BK;
             if (p.N >= Prod::BW && nwhole >= 1) {
                 const bool whole_ok =
-                    full_mn ? produce_blocks<kLoadFull>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nwhole, limit)
-                            : produce_blocks<kLoadEdge>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nwhole, limit);
+                    full_mn ? produce_blocks<kLoadFull>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nwhole, jb, limit)
+                            : produce_blocks<kLoadEdge>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nwhole, jb, limit);
                 if (!whole_ok) return;
                 if (nwhole < nkb) {  // the partial last k-block
                     const long long j = nwhole;
                     load_block<0, kLoadGeneral>(t, ptid, p, m0, n0, kbeg + j * BK, kend, full_mn);
-                    const int slot = (int)(j % NS);
-                    const unsigned round = (unsigned)(j / NS);
+                    const int slot = (int)((jb + j) % NS);
+                    const unsigned round = (unsigned)((jb + j) / NS);
                     if (round > 0 && !wait_count(&sh.freed[slot], CWAVES * round, &sh.abort_flag, limit)) return;
                     t.template store_as<0>(ptid, ring + slot * STAGE);
                     signal(&sh.ready[slot]);
                 }
             } else {
-                (void)produce_blocks<kLoadGeneral>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, limit);
+                (void)produce_blocks<kLoadGeneral>(t, ptid, ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, jb, limit);
             }
         }
     }
 
     __device__ __forceinline__ static void consumer(char* ring, Shared& sh, float* dst, const GemmParams& p,
-                                                    long long m0, long long n0, long long nkb, unsigned limit) {
+                                                    long long m0, long long n0, long long nkb, long long jb,
+                                                    unsigned limit) {
         Cons t;
 #pragma unroll
         for (int i = 0; i < Cons::TM; ++i)
@@ -215,8 +222,8 @@ struct Feed {
                 for (int e = 0; e < 16; ++e) t.acc[i][j][e] = 0.f;
         bool ok = true;
         for (long long j = 0; j < nkb; ++j) {
-            const int slot = (int)(j % NS);
-            const unsigned round = (unsigned)(j / NS);
+            const int slot = (int)((jb + j) % NS);
+            const unsigned round = (unsigned)((jb + j) / NS);
             if (!wait_count(&sh.ready[slot], PWAVES * (round + 1), &sh.abort_flag, limit)) {
                 ok = false;
                 break;
@@ -261,11 +268,139 @@ __global__ __launch_bounds__(64 * (CWM * CWN + PWM * PWN)) void gemm_feed_kernel
     const long long nkb = (kend - kbeg + BK - 1) / BK;
     const bool full_mn = (m0 + BM <= p.M) && (n0 + BN <= p.N);
     if (threadIdx.x < F::CT) {
-        F::consumer(ring, sh, p.C + split * p.c_split_stride, p, m0, n0, nkb, spin_limit);
+        F::consumer(ring, sh, p.C + split * p.c_split_stride, p, m0, n0, nkb, 0, spin_limit);
     } else if (full_mn && (kend - kbeg) % BK == 0) {
-        F::template producer<true>(ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, spin_limit);
+        F::template producer<true>(ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, 0, spin_limit);
     } else {
-        F::template producer<false>(ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, spin_limit);
+        F::template producer<false>(ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, 0, spin_limit);
+    }
+}
+
+// ---- grouped persistent launch on the feed (split modes) ---------------------------
+//
+// The persistent workgroups of the grouped launch (grouped.hip) made of
+// consumer and producer waves: each workgroup claims work items from the
+// launch's counter and runs each one through its ring. The ring's counters
+// keep counting across items: jb, the number of k-blocks the workgroup has
+// passed through the ring so far, is the same in every wave (every wave runs
+// every item's blocks), so the workgroup's block jb + j goes to slot
+// (jb + j) % NS in round (jb + j) / NS. The kernel serves the direct grouped
+// kernel's geometries 1..4 (64 x 128, 64 x 256, 128 x 128, 128 x 256) with the
+// same work items, so every element gets the products of the direct launch
+// in the same order. All geometries of a mode share the wave counts (eight
+// consumer and four producer waves, 168 registers per lane, of which the
+// compiler spills 5 bytes in split3 and 22 in split6), the ring slot count
+// and one LDS ring sized for the largest. Measured against the direct grouped
+// kernel on one
+// MI355X, bit-identical every time:
+//   the 256-problem batch (M ~ U{1..512}, N in {512, 2112, 4096, 7168}, K in
+//     {512, 1536, 7168, 16384}; nearly every problem on 128 x 256 tiles):
+//     split3 12.57 -> 12.25 ms, split6 18.64 -> 17.20 ms;
+//   other batches, split3 / split6: K in {256, 512} 1.01x / 1.04x; M <= 128
+//     with K in {8192, 16384} 1.07x / 1.11x; M <= 64 1.06x / 1.05x; M in
+//     256..512 1.05x / 1.08x; N in {64, 96, 130, 200} 1.17x / 1.20x; eight
+//     problems 1.13x / 1.25x; sixteen with K = 16384 1.07x / 1.07x.
+// Variants measured on the 256-problem batch and not kept: eight producer
+// waves leave 128 registers per lane and spill (split3 13.3-16.9 ms, split6
+// 20.7-21.8 ms); three slots with two k-blocks of loads in flight 12.3 ms on
+// the 128 x 256 geometry alone but spilling with all four; 16-deep k-blocks
+// (split3 21.6 ms, split6 20.1 ms); everything on 128 x 128 tiles with the
+// standalone feed's 64-deep blocks (twice the items; split3 15.0 ms).
+template <int MODE>
+struct GroupedFeed;
+template <>
+struct GroupedFeed<1> {
+    template <int BM, int BN>
+    using G = Feed<1, BM, BN, 32, 2, 1, 2, 4, 2, 2>;
+};
+template <>
+struct GroupedFeed<2> {
+    template <int BM, int BN>
+    using G = Feed<2, BM, BN, 32, 2, 1, 2, 4, 2, 2>;
+};
+
+template <int MODE>
+constexpr int grouped_feed_nt() {
+    using F = typename GroupedFeed<MODE>::template G<128, 256>;
+    return F::CT + F::PT;
+}
+
+// One work item of problem q (item `local` of the problem) on geometry F;
+// returns its k-block count.
+template <class F>
+__device__ __forceinline__ long long grouped_feed_item(const GroupProblem& q, int local, char* ring,
+                                                       typename F::Shared& sh, long long jb, unsigned limit) {
+    const int ntiles = q.tiles_m * q.tiles_n;
+    const int split = local / ntiles;
+    int tm, tn;
+    grouped_tile(local - split * ntiles, q.tiles_m, q.tiles_n, tm, tn);
+    const long long kbeg = split * q.k_len;
+    const long long kend = min(q.K, kbeg + q.k_len);
+    const long long m0 = (long long)tm * F::kBM, n0 = (long long)tn * F::kBN;
+    const long long nkb = (kend - kbeg + F::kBK - 1) / F::kBK;
+    const bool full_mn = (m0 + F::kBM <= q.M) && (n0 + F::kBN <= q.N);
+    GemmParams p;
+    p.A = q.A;
+    p.B = q.B;
+    p.C = nullptr;
+    p.M = q.M;
+    p.N = q.N;
+    p.K = q.K;
+    p.lda = q.K;
+    p.ldb = q.N;
+    p.ldc = q.N;
+    p.k_len = q.k_len;
+    p.c_split_stride = 0;
+    p.splits = q.splits;
+    p.tiles_m = q.tiles_m;
+    p.tiles_n = q.tiles_n;
+    float* dst = q.splits > 1 ? q.P + split * q.M * q.N : q.C;
+    if (threadIdx.x < F::CT)
+        F::consumer(ring, sh, dst, p, m0, n0, nkb, jb, limit);
+    else if (full_mn && (kend - kbeg) % F::kBK == 0)
+        F::template producer<true>(ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, jb, limit);
+    else
+        F::template producer<false>(ring, sh, p, m0, n0, kbeg, kend, full_mn, nkb, jb, limit);
+    return nkb;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(grouped_feed_nt<MODE>()) void grouped_feed_kernel(GroupLaunch g, unsigned spin_limit) {
+    using GF = GroupedFeed<MODE>;
+    using G1 = typename GF::template G<64, 128>;
+    using G2 = typename GF::template G<64, 256>;
+    using G3 = typename GF::template G<128, 128>;
+    using G4 = typename GF::template G<128, 256>;
+    static_assert(G1::CT == G4::CT && G1::PT == G4::PT && G2::CT == G4::CT && G2::PT == G4::PT &&
+                      G3::CT == G4::CT && G3::PT == G4::PT,
+                  "one workgroup shape");
+    static_assert(G4::RING >= G1::RING && G4::RING >= G2::RING && G4::RING >= G3::RING, "ring of the largest");
+    static_assert(kGroupedMaxCfg == 4 && grouped_min_cfg(MODE) == 1, "geometries 1..4");
+    __shared__ __attribute__((aligned(16))) char ring[G4::RING];
+    __shared__ FeedShared<G4::kNS> sh;
+    __shared__ int s_item;
+    if (threadIdx.x < G4::kNS) {
+        sh.ready[threadIdx.x] = 0u;
+        sh.freed[threadIdx.x] = 0u;
+    }
+    if (threadIdx.x == 0) sh.abort_flag = 0u;
+    long long jb = 0;
+    // Every wave leaves when the list is exhausted; an aborted handshake only
+    // turns the remaining items' waits into immediate failures (NaN tiles).
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(g.counter, 1u);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();  // every wave has read s_item before it is overwritten
+        if (item >= g.total_items) break;
+        const GroupProblem& q = g.problems[g.item_problem[item]];
+        const int local = item - q.item_begin;
+        switch (q.cfg) {
+            case 1: jb += grouped_feed_item<G1>(q, local, ring, sh, jb, spin_limit); break;
+            case 2: jb += grouped_feed_item<G2>(q, local, ring, sh, jb, spin_limit); break;
+            case 3: jb += grouped_feed_item<G3>(q, local, ring, sh, jb, spin_limit); break;
+            default: jb += grouped_feed_item<G4>(q, local, ring, sh, jb, spin_limit); break;
+        }
     }
 }
 
@@ -384,6 +519,42 @@ hipError_t launch_gemm_feed(int variant, const GemmParams& p, hipStream_t stream
         case 8: return launch_feed_cfg<2, 32, 2, 2, 128, 128, 2, 4, 2, 2>(p, stream);
         default: return hipErrorInvalidValue;
     }
+}
+
+bool grouped_feed(int mode, int setting) { return mode != 0 && setting != 0; }
+
+int grouped_feed_threads(int mode) {
+    return mode == 1 ? grouped_feed_nt<1>() : (mode == 2 ? grouped_feed_nt<2>() : 0);
+}
+
+hipError_t launch_grouped_feed(int mode, const GroupLaunch& g, int grid, hipStream_t stream) {
+    if (grid <= 0) return hipErrorInvalidValue;
+    switch (mode) {
+        case 1:
+            hipLaunchKernelGGL(grouped_feed_kernel<1>, dim3(grid), dim3(grouped_feed_nt<1>()), 0, stream, g,
+                               spin_limit());
+            break;
+        case 2:
+            hipLaunchKernelGGL(grouped_feed_kernel<2>, dim3(grid), dim3(grouped_feed_nt<2>()), 0, stream, g,
+                               spin_limit());
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int grouped_feed_blocks_per_cu(int mode) {
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+    if (mode == 1)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_feed_kernel<1>, grouped_feed_nt<1>(), 0);
+    else if (mode == 2)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_feed_kernel<2>, grouped_feed_nt<2>(), 0);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return 1;
+    }
+    return n > 0 ? n : 1;
 }
 
 }  // namespace mmsk

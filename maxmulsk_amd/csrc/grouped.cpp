@@ -27,6 +27,7 @@ struct maxmulsk_grouped_plan_s {
     int total_items = 0;
     int reduce_blocks = 0;
     int grid = 0;
+    bool feed = false;  // split modes on the decoupled feed (grouped_feed_kernel)
     double flops = 0;
     std::vector<std::pair<float*, size_t>> zero_fills;  // K == 0 problems
     GroupProblem* d_problems = nullptr;
@@ -128,7 +129,7 @@ hipError_t enqueue(const maxmulsk_grouped_plan_s* p, hipStream_t s) {
         if (e == hipSuccess) e = launch_fill_zero_f32(z.first, z.second, s);
     if (e == hipSuccess && p->total_items > 0) {
         GroupLaunch g{p->d_problems, p->d_item_problem, p->total_items, p->d_counter};
-        e = launch_grouped(p->mode, g, p->grid, s);
+        e = p->feed ? launch_grouped_feed(p->mode, g, p->grid, s) : launch_grouped(p->mode, g, p->grid, s);
     }
     if (e == hipSuccess && p->reduce_blocks > 0)
         e = launch_grouped_reduce(p->d_problems, p->d_block_problem, p->reduce_blocks, s);
@@ -153,6 +154,9 @@ maxmulsk_status build(const maxmulsk_gemm* problems, size_t n, int mode, const m
 
     const std::vector<maxmulsk_plan> defaults =
         plans ? std::vector<maxmulsk_plan>() : batch_defaults(problems, n, mode, true);
+
+    // Split modes run on the decoupled feed under the current feed setting.
+    const bool feed = grouped_feed(mode, feed_mode());
 
     // Validate every problem before anything is allocated or run.
     int dev = -1;
@@ -183,6 +187,7 @@ maxmulsk_status build(const maxmulsk_gemm* problems, size_t n, int mode, const m
 
     auto* p = new maxmulsk_grouped_plan_s();
     p->mode = mode;
+    p->feed = feed;
     p->n = n;
     if (dev < 0) {  // nothing to run
         *out = p;
@@ -296,7 +301,8 @@ maxmulsk_status build(const maxmulsk_gemm* problems, size_t n, int mode, const m
     hipDeviceProp_t prop;
     int cus = kNumCUs;
     if (hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-    p->grid = std::max(1, std::min(p->total_items, cus * grouped_blocks_per_cu(mode)));
+    p->grid = std::max(1, std::min(p->total_items,
+                                   cus * (feed ? grouped_feed_blocks_per_cu(mode) : grouped_blocks_per_cu(mode))));
 
     if (capture) {
         // Capture needs a created stream; the replay may go to any stream.
@@ -365,6 +371,8 @@ extern "C" maxmulsk_status maxmulsk_grouped_info(maxmulsk_grouped_plan plan, siz
     if (used_graph) *used_graph = plan->exec != nullptr;
     return MAXMULSK_OK;
 }
+
+extern "C" int maxmulsk_grouped_uses_feed(maxmulsk_grouped_plan plan) { return plan && plan->feed ? 1 : 0; }
 
 extern "C" void maxmulsk_grouped_destroy(maxmulsk_grouped_plan plan) { free_plan(plan); }
 

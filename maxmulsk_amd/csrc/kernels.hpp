@@ -188,6 +188,15 @@ constexpr int grouped_min_cfg(int mode) { return mode == 0 ? 0 : 1; }
 constexpr int grouped_threads(int mode) { return mode == 0 ? 256 : 512; }
 // Workgroups of the persistent kernel that fit on one CU at once.
 int grouped_blocks_per_cu(int mode);
+// The split modes' persistent kernel on the decoupled feed (feed.hip): the
+// same work items and geometries (1..4) as grouped_kernel, with producer and
+// consumer waves per workgroup. grouped_feed tells whether a launch of `mode`
+// built under feed setting `setting` takes it: every split-mode launch unless
+// the setting is 0 (direct).
+bool grouped_feed(int mode, int setting);
+int grouped_feed_threads(int mode);
+hipError_t launch_grouped_feed(int mode, const GroupLaunch& g, int grid, hipStream_t stream);
+int grouped_feed_blocks_per_cu(int mode);
 // Split-K reduce of every split problem: reduce block b serves problem
 // block_problem[b], elements [(b - reduce_begin) * 1024, +1024) of its M x N.
 hipError_t launch_grouped_reduce(const GroupProblem* problems, const int* block_problem,

@@ -397,6 +397,16 @@ def test_lds_patterns_of_the_decoupled_feed(mode, rows, bk, threads, b_cols):
     _check_tile_lds_patterns(mode, rows, rows, bk, threads, b_cols)
 
 
+@pytest.mark.parametrize("mode,bm,bn,b_cols", [
+    (m, bm, bn, 2 if bn == 128 else 4) for m in (1, 2) for bm, bn in ((64, 128), (64, 256), (128, 128), (128, 256))])
+def test_lds_patterns_of_the_grouped_feed(mode, bm, bn, b_cols):
+    """The grouped launch on the decoupled feed stages the direct grouped
+    kernel's four split-mode geometries into 32-deep slots with four producer
+    waves (B columns per unit as the direct tile takes for four waves)."""
+    assert mm.tile_info(1 if bn == 128 and bm == 64 else (2 if bm == 64 else (3 if bn == 128 else 4)), mode)["bk"] == 32
+    _check_tile_lds_patterns(mode, bm, bn, 32, 256, b_cols)
+
+
 def test_feed_variant_rule():
     """Which kernel a launch takes under each feed setting (pure host code):
     the automatic rule feeds launches of at most one workgroup per CU: 64 x 64
