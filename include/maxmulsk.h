@@ -333,7 +333,7 @@ maxmulsk_status maxmulsk_probe_mfma(int mode, int random_operands, int iteration
  * the direct tile elsewhere. Both feeds give the same C (same K
  * slices, products and order), except that an exact -0 may come out as +0.
  * Split-mode grouped launches run on the feed under DECOUPLED and AUTO (it
- * measured 1.01-1.25x faster on every batch tried), on the direct grouped
+ * measured 1.03-1.37x faster on every batch tried), on the direct grouped
  * kernel under DIRECT; the setting is read when the grouped plan is created.
  * Plain grouped launches keep their own kernel. Counterpart of the reference's Trio
  * engine, one compute thread fed by separate packing threads through rings

@@ -287,36 +287,40 @@ __global__ __launch_bounds__(64 * (CWM * CWN + PWM * PWN)) void gemm_feed_kernel
 // (jb + j) % NS in round (jb + j) / NS. The kernel serves the direct grouped
 // kernel's geometries 1..4 (64 x 128, 64 x 256, 128 x 128, 128 x 256) with the
 // same work items, so every element gets the products of the direct launch
-// in the same order. All geometries of a mode share the wave counts (eight
-// consumer and four producer waves, 168 registers per lane, of which the
-// compiler spills 5 bytes in split3 and 22 in split6), the ring slot count
-// and one LDS ring sized for the largest. Measured against the direct grouped
-// kernel on one
-// MI355X, bit-identical every time:
+// in the same order. All geometries of a mode share the wave counts (four
+// consumer waves in a 2 x 2 layout, so a 128 x 256 tile gives each 64 x 128
+// and every fragment read from LDS feeds four MFMAs; four producer waves; 250
+// VGPRs per lane in split3, 256 in split6 with a 2-byte spill), the ring slot
+// count and one LDS ring sized for the largest. Measured against the direct
+// grouped kernel on one MI355X, bit-identical every time:
 //   the 256-problem batch (M ~ U{1..512}, N in {512, 2112, 4096, 7168}, K in
 //     {512, 1536, 7168, 16384}; nearly every problem on 128 x 256 tiles):
-//     split3 12.57 -> 12.25 ms, split6 18.64 -> 17.20 ms;
-//   other batches, split3 / split6: K in {256, 512} 1.01x / 1.04x; M <= 128
-//     with K in {8192, 16384} 1.07x / 1.11x; M <= 64 1.06x / 1.05x; M in
-//     256..512 1.05x / 1.08x; N in {64, 96, 130, 200} 1.17x / 1.20x; eight
-//     problems 1.13x / 1.25x; sixteen with K = 16384 1.07x / 1.07x.
-// Variants measured on the 256-problem batch and not kept: eight producer
-// waves leave 128 registers per lane and spill (split3 13.3-16.9 ms, split6
-// 20.7-21.8 ms); three slots with two k-blocks of loads in flight 12.3 ms on
-// the 128 x 256 geometry alone but spilling with all four; 16-deep k-blocks
-// (split3 21.6 ms, split6 20.1 ms); everything on 128 x 128 tiles with the
-// standalone feed's 64-deep blocks (twice the items; split3 15.0 ms).
+//     split3 12.77 -> 11.69 ms, split6 18.90 -> 17.12 ms;
+//   other batches, split3 / split6: K in {256, 512} 1.03x / 1.05x; M <= 128
+//     with K in {8192, 16384} 1.20x / 1.15x; M <= 64 1.26x / 1.12x; M in
+//     256..512 1.08x / 1.10x; N in {64, 96, 130, 200} 1.37x / 1.26x; eight
+//     problems 1.23x / 1.31x; sixteen with K = 16384 1.12x / 1.07x.
+// Variants measured on the 256-problem batch and not kept (split3 / split6,
+// same session as the kept 11.69 / 17.12 ms unless noted): eight consumer
+// waves of 64 x 64 with four producers, 12.40 / 17.50 ms (twice the LDS
+// fragment reads per product); eight consumers and eight producers leave 128
+// registers per lane and spill (13.3-16.9 / 20.7-21.8 ms, other sessions);
+// four consumers with one k-block of loads in flight in split3, 1.05x over
+// direct against 1.09x; eight producers with four consumers spill 857 bytes
+// in split6 (0.14x); 16-deep k-blocks (21.6 / 20.1 ms, eight consumers);
+// everything on 128 x 128 tiles with the standalone feed's 64-deep blocks
+// (twice the items; split3 15.0 ms).
 template <int MODE>
 struct GroupedFeed;
 template <>
 struct GroupedFeed<1> {
     template <int BM, int BN>
-    using G = Feed<1, BM, BN, 32, 2, 1, 2, 4, 2, 2>;
+    using G = Feed<1, BM, BN, 32, 2, 2, 2, 2, 2, 2>;
 };
 template <>
 struct GroupedFeed<2> {
     template <int BM, int BN>
-    using G = Feed<2, BM, BN, 32, 2, 1, 2, 4, 2, 2>;
+    using G = Feed<2, BM, BN, 32, 2, 1, 2, 2, 2, 2>;
 };
 
 template <int MODE>
