@@ -206,16 +206,26 @@ def main() -> int:
             source = "MAXMULSK_PLAN" if os.environ.get("MAXMULSK_PLAN") else "autotune cache"
             r = {"tflops": tf, "ms": ms, "round_ms": times[n], "spread_pct": spread_pct(times[n]),
                  "plan": list(plans[n]), "plan_source": source, "errors": e}
-            if plans[n][0] == 256:
-                # the 256-wide path packs both operands on every call; time its
-                # GEMM kernel alone on pre-packed operands
+            # Does the call pack its operands first (the 256 x 256 packed path)?
+            # The library's launch counters say.
+            mm.reset_phase_counters()
+            make_call(dA, dB, dC, M, K, N, m)()
+            torch.cuda.synchronize()
+            counts = mm.phase_counters()
+            r["packs_operands"] = counts["pack"] > 0
+            if r["packs_operands"]:
+                # time its GEMM kernel alone on pre-packed operands
                 kms, ksp = packed_kernel_ms(dA, dB, dC, M, K, N, m, plans[n][1], args.steps)
                 r["gemm_kernel_ms"] = kms
                 r["gemm_kernel_spread_pct"] = ksp
                 r["packed_operands_tflops"] = 2.0 * M * N * K / (kms * 1e-3) / 1e12
                 r["pack_share_of_call"] = 1.0 - kms / ms
+                r["gemm_kernel"] = f"gemm_packed_kernel<{m}>"
             else:
+                # one GEMM launch per call (plus the reduce with split-K)
                 r["gemm_kernel_ms"] = ms
+                r["gemm_kernel"] = (f"gemm_kernel<{m}, 256, 256> (direct tile, operands split on the fly)"
+                                    if plans[n][0] == 256 and M > 128 else "gemm_kernel (direct tile)")
             res[n] = r
             log(f"4096^3 {n}: {ms:.4f} ms (spread {r['spread_pct']:.1f}%) {tf:.1f} TFLOP/s plan {plans[n]} "
                 f"kernel {r['gemm_kernel_ms']:.4f} ms err {e}")
@@ -392,11 +402,13 @@ def main() -> int:
         issued = products * 2.0 * 4096 ** 3
         kms = r["gemm_kernel_ms"]
         achieved = issued / (kms * 1e-3) / 1e12
+        how = ("kernel timed alone on pre-packed operands with HIP events on its stream" if r["packs_operands"]
+               else "the call is this one launch, timed with HIP events on its stream")
         line["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK, "unit": "TFLOP/s",
-                            "frac": achieved / BF16_PEAK, "traffic": pmc_traffic(head, r["plan"]),
-                            "kernel_ms": kms,
-                            "note": f"BF16 MFMA FLOPs = {products} x 2MNK per launch; kernel timed alone on "
-                                    "pre-packed operands with HIP events on its stream"}
+                            "frac": achieved / BF16_PEAK,
+                            "traffic": pmc_traffic("packed" if r["packs_operands"] else "direct256", head, r["plan"]),
+                            "kernel": r["gemm_kernel"], "kernel_ms": kms,
+                            "note": f"BF16 MFMA FLOPs = {products} x 2MNK per launch; {how}"}
         line["roofline"]["frac_of_random_operand_ceiling"] = (
             achieved / out["mfma_probe"]["bf16_32x32x16"]["random_operands_tflops"])
         line["mfma_probe"] = out["mfma_probe"]
@@ -435,11 +447,12 @@ def errors_torch(dA, dB, dC, K: int) -> dict:
             "within_fp32_bound": ratio <= 1.0}
 
 
-def pmc_traffic(mode: str, plan) -> int | None:
-    """HBM bytes per launch of the packed GEMM kernel at 4096^3, from the
-    committed rocprofv3 counter summary (FETCH_SIZE x its calibration factor +
-    WRITE_SIZE). None when no summary matches this mode and plan."""
-    path = os.path.join(ROOT, "profiles", f"gemm_packed_{mode}_4096_pmc.txt")
+def pmc_traffic(kernel: str, mode: str, plan) -> int | None:
+    """HBM bytes per launch of the 4096^3 GEMM kernel (kernel "packed" or
+    "direct256"), from the committed rocprofv3 counter summary (FETCH_SIZE x
+    its calibration factor + WRITE_SIZE). None when no summary matches this
+    kernel, mode and plan."""
+    path = os.path.join(ROOT, "profiles", f"gemm_{kernel}_{mode}_4096_pmc.txt")
     if not os.path.exists(path):
         return None
     vals = {}

@@ -95,7 +95,12 @@ static hipError_t launch_mode(int cfg, const GemmParams& p, hipStream_t stream) 
             }
             return launch_one<MODE, 3>(p, stream);
         case 4: return launch_one<MODE, 4>(p, stream);
-        default: return hipErrorInvalidValue;  // geometry 5 runs on the packed path
+        case 5:
+            // 256 x 256 in the split modes on whole tiles and k-blocks
+            // (packed_path); every other call of geometry 5 runs packed.
+            if constexpr (MODE != 0) return launch_one<MODE, 5>(p, stream);
+            return hipErrorInvalidValue;
+        default: return hipErrorInvalidValue;
     }
 }
 

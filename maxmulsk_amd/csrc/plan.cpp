@@ -34,6 +34,11 @@ int cfg_for(int tile, size_t M) {
     }
 }
 
+bool packed_path(int mode, size_t M, size_t N, size_t K) {
+    if (mode == 0) return true;
+    return !(M % 256 == 0 && N % 256 == 0 && K % (size_t)packed_bk(mode) == 0);
+}
+
 bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* out, int max_cfg, int min_cfg) {
     if (mode < 0 || mode > 2) return false;
     if (M == 0 || K == 0 || N == 0) return false;
@@ -49,8 +54,8 @@ bool resolve(size_t M, size_t K, size_t N, int mode, maxmulsk_plan p, Resolved* 
     r.cfg = cfg;
     r.bm = tc.bm;
     r.bn = tc.bn;
-    r.packed = (cfg == kPackedCfg);
-    r.bk = r.packed ? packed_bk(mode) : tc.bk[mode];
+    r.packed = cfg == kPackedCfg && packed_path(mode, M, N, K);
+    r.bk = cfg == kPackedCfg ? packed_bk(mode) : tc.bk[mode];
     const long long kblocks = ((long long)K + r.bk - 1) / r.bk;
     long long per = (kblocks + p.split_k - 1) / p.split_k;  // k-blocks per slice
     if (per < 1) per = 1;

@@ -25,6 +25,17 @@ int candidate_index(maxmulsk_plan p);
 // edge is the tile edge; the M edge shrinks to 64 or 128 for narrow M.
 int cfg_for(int tile, size_t M);
 
+// Whether a call on the 256 x 256 geometry packs its operands first (the
+// packed GEMM) or splits them on the fly in the direct tile kernel. The split
+// modes take the direct tile when every tile and k-block is whole (M and N
+// multiples of 256, K a multiple of the k-block depth): with the pack pass
+// gone it measured 1.04-1.19x faster in split3 and 1.06-1.19x in split6 on
+// such shapes (4096^3 split3 367.7 -> 345.8 us, split6 640.3 -> 584.7 us),
+// bit-identical. On ragged shapes its element-wise edge loads lose (split3
+// 0.76-1.08x, split6 0.94-1.07x), and plain mode is 1-7% slower direct, so
+// those keep the pack.
+bool packed_path(int mode, size_t M, size_t N, size_t K);
+
 // A plan resolved against a shape: geometry, K slicing, grid and workspace.
 struct Resolved {
     int cfg;
@@ -33,8 +44,9 @@ struct Resolved {
     long long k_len;     // K elements per slice, a multiple of bk
     int tiles_m, tiles_n;
     size_t ws_bytes;     // split-K partials + packed operands
-    // Packed path (256 x 256 geometry): operands rewritten once into MFMA
-    // fragments in the workspace, after the partials.
+    // Packed path (256 x 256 geometry unless packed_path says otherwise):
+    // operands rewritten once into MFMA fragments in the workspace, after the
+    // partials.
     bool packed;
     long long kblocks;   // packed k-blocks (packed_bk(mode) deep)
     int kgroups;         // fragments per row block along K

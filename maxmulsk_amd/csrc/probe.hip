@@ -201,7 +201,7 @@ extern "C" maxmulsk_status maxmulsk_probe_stream(size_t M, size_t K, size_t N, c
                                                  void* stream, double* us) {
     Resolved r{};
     if (!plan || !us || calls <= 0 || K % 4 != 0 || N % 4 != 0 ||
-        !resolve(M, K, N, MAXMULSK_MODE_SPLIT3, *plan, &r) || r.packed)
+        !resolve(M, K, N, MAXMULSK_MODE_SPLIT3, *plan, &r) || r.cfg == kPackedCfg)
         return fail(MAXMULSK_ERR_UNSUPPORTED, "probe_stream: direct-tile plan, K and N multiples of 4, calls > 0");
     int dev = -1;
     if (usable_devices() == 0 || hipGetDevice(&dev) != hipSuccess || !device_usable(dev)) {

@@ -160,10 +160,17 @@ def test_workspace_bytes():
     # 256-wide tiles on more than 128 rows take the packed path: the workspace
     # holds A and B rewritten as fragments (FP32 copy, or 2 / 3 BF16 terms),
     # padded to whole 256-row tiles and whole packed k-blocks (32 deep in plain
-    # and split3, 16 in split6)
+    # and split3, 16 in split6) ...
     assert mm.workspace_bytes(4096, 4096, 4096, 0, (256, 1)) == 2 * 4096 * 4096 * 4
-    assert mm.workspace_bytes(4096, 4096, 4096, 1, (256, 1)) == 2 * 2 * 4096 * 4096 * 2
-    assert mm.workspace_bytes(4096, 4096, 4096, 2, (256, 1)) == 2 * 3 * 4096 * 4096 * 2
+    assert mm.workspace_bytes(4095, 4096, 4096, 1, (256, 1)) == 2 * 2 * 4096 * 4096 * 2
+    assert mm.workspace_bytes(4096, 4096, 4000, 2, (256, 1)) == 2 * 3 * 4096 * 4096 * 2
+    assert mm.workspace_bytes(4096, 4090, 4096, 2, (256, 1)) == 2 * 3 * 4096 * 4096 * 2  # K % 16 != 0
+    # ... except in the split modes on whole tiles and k-blocks, where the
+    # direct 256 x 256 tile splits on the fly: split-K partials only
+    assert mm.workspace_bytes(4096, 4096, 4096, 1, (256, 1)) == 0
+    assert mm.workspace_bytes(4096, 4096, 4096, 2, (256, 1)) == 0
+    assert mm.workspace_bytes(4096, 4096, 4096, 1, (256, 2)) == 2 * 4096 * 4096 * 4
+    assert mm.workspace_bytes(512, 4080, 768, 2, (256, 4)) == 4 * 512 * 768 * 4
     assert mm.workspace_bytes(300, 20, 300, 1, (256, 1)) == 2 * (2 * 512 * 32 * 2)
     assert mm.workspace_bytes(300, 40, 300, 0, (256, 1)) == 2 * (512 * 64 * 4)
     assert mm.workspace_bytes(300, 40, 300, 1, (256, 1)) == 2 * (2 * 512 * 64 * 2)

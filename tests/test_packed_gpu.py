@@ -1,8 +1,9 @@
 """Pack-once operands (maxmulsk_pack / maxmulsk_sgemm_packed) through the C ABI:
 the packed product must equal, bit for bit, the per-call path of
 maxmulsk_sgemm_ex with a 256-wide plan on more than 128 rows (the same
-fragments and kernel, packed on every call), and stay within the FP32 bound of
-the FP64 oracle."""
+fragments and kernel, packed on every call; in the split modes on whole tiles
+and k-blocks, the direct 256 x 256 tile that splits on the fly), and stay
+within the FP32 bound of the FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -78,3 +79,37 @@ def test_packed_rejects_mismatch():
     assert mm.lib().maxmulsk_pack(host.ctypes.data, 256, 64, 0, 1, None, mm.C.byref(h)) == mm.ERR_UNSUPPORTED
     for p in (pa, pb, pb2):
         p.close()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mnk,split", [((512, 768, 1024), 1), ((768, 512, 4096), 4), ((256, 256, 96), 2),
+                                       ((1024, 1024, 1024), 32), ((2048, 1024, 512), 1)],
+                         ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else str(v))
+def test_whole_tile_split_calls_skip_the_pack(mode, mnk, split):
+    """Split-mode 256-wide plans on whole tiles and k-blocks run the direct
+    256 x 256 tile, which splits the operands on the fly: no pack launch, one
+    GEMM (plus the reduce with split-K), and C bit-identical to the packed
+    kernel on pre-packed operands, within each mode's bound: the FP32 bound
+    plus split3's K-independent 3 * 2^-16 * |a||b| per product (a ratio of
+    1 + 768 / K) and split6's 2^-24-scale terms (1 + 4 / K)."""
+    torch = torch_cuda()
+    M, N, K = mnk
+    A = orc.uniform(81, (M, K))
+    B = orc.log_uniform(82, (K, N))
+    dA, dB, dC = Guarded(M * K, A), Guarded(K * N, B), Guarded(M * N, SENTINEL)
+    torch.cuda.synchronize()
+    print(f"direct 256 x 256: mode {mode} {M}x{N}x{K} split {split}", flush=True)
+    assert mm.sgemm_ex(dA.ptr, dB.ptr, dC.ptr, M, K, N, mode=mode, plan=(256, split)) == mm.OK  # warms the pool
+    mm.reset_phase_counters()
+    assert mm.sgemm_ex(dA.ptr, dB.ptr, dC.ptr, M, K, N, mode=mode, plan=(256, split)) == mm.OK
+    torch.cuda.synchronize()
+    c = mm.phase_counters()
+    assert (c["pack"], c["compute"], c["reduce"]) == (0, 1, int(split > 1)), c
+    assert dA.guards_intact() and dB.guards_intact() and dC.guards_intact()
+    Cd = dC.numpy((M, N))
+    Cp, pb = packed_gemm(A, B, mode, split)
+    pb.close()
+    assert np.array_equal(Cd, Cp)
+    e = full_check(A, B, Cd)
+    limit = {1: 1.0 + 768.0 / K, 2: 1.0 + 4.0 / K}[mode]
+    assert e["nonfinite"] == 0 and e["bound_ratio"] <= limit, e
