@@ -126,10 +126,12 @@ maxmulsk_status maxmulsk_sgemm_noalloc(const float* A, const float* B, float* C,
 /* Pack once, multiply many: an operand rewritten into the MFMA fragment
  * layout (split into its BF16 terms in the split modes), owned by the library.
  * Operand 0 is an A (rows = M, row-major M x K), operand 1 a B (rows = N,
- * row-major K x N). Packed operands feed the 256 x 256 tile path, the one
- * maxmulsk_sgemm_ex takes for 256-wide plans on more than 128 rows, which packs
- * both operands on every call; a caller that reuses an operand (weights)
- * packs it once. Counterpart of the reference's B panel packed once per
+ * row-major K x N). Packed operands feed the packed 256 x 256 GEMM kernel,
+ * which maxmulsk_sgemm_ex runs for 256-wide plans on more than 128 rows in
+ * plain mode and on ragged split-mode shapes, packing both operands on every
+ * call (split-mode calls on whole 256 x 256 tiles split on the fly instead); a
+ * caller that reuses an operand (weights) packs it once. Results are the same
+ * bit for bit on every path. Counterpart of the reference's B panel packed once per
  * (Kc, Nc) block (sme/v4/sme-1x4-kcout-ncblock-apack4za.cpp:498-503). Device
  * pointers only. */
 typedef struct maxmulsk_packed_s* maxmulsk_packed;
