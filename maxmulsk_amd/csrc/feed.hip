@@ -1,4 +1,7 @@
-// Decoupled producer/consumer feed for the 64 x 64 tile (geometry 0).
+// Decoupled producer/consumer feed: the direct tile's work split between
+// waves that only multiply and waves that only stage operands, for the 64 x 64
+// tile (geometry 0), the split modes' 128 x 128 tile (geometry 3), and the
+// split modes' grouped persistent launch (geometries 1..4).
 //
 // Counterpart of the reference's Trio (sme/trio/trio_sme.cpp:274-334,
 // sme/trio/trio_packers.cpp:31-107): one compute thread that only runs the
@@ -7,7 +10,7 @@
 // sequence counters; same blocking and product order as the base kernel, so
 // the result is bit-identical to it.
 //
-// On gfx950 the threads are the waves of one 512-thread workgroup:
+// On gfx950 the threads are the waves of one workgroup, e.g. for 64 x 64:
 //   waves 0..3 (consumers): the 2 x 2 wave layout of the direct 64 x 64 tile;
 //     they only read LDS fragments, run the MFMAs and write the tile (or its
 //     split-K partial);
@@ -18,13 +21,13 @@
 // Slot j % NS holds k-block j. Per slot two counters in LDS count waves:
 //   ready[s]  += 1 by each producer wave once its stores of a block are done;
 //   freed[s]  += 1 by each consumer wave once its reads of a block are done.
-// A producer fills k-block j after freed[j % NS] reaches 4 * (j / NS); a
-// consumer reads it after ready[j % NS] reaches 4 * (j / NS + 1). Counts only
-// grow, so no reset is needed and no slot is read before it is written or
-// overwritten before it is read. Every wait is bounded: past the bound the
-// wave raises a workgroup abort flag, every wave leaves its loop, and the
-// consumers write NaN to their tile, so a broken handshake shows up as wrong
-// output and never as a hang.
+// With P producer and C consumer waves, a producer fills k-block j after
+// freed[j % NS] reaches C * (j / NS); a consumer reads it after
+// ready[j % NS] reaches P * (j / NS + 1). Counts only grow, so no reset is
+// needed and no slot is read before it is written or overwritten before it is
+// read. Every wait is bounded: past the bound the wave raises a workgroup
+// abort flag, every wave leaves its loop, and the consumers write NaN to their
+// tile, so a broken handshake shows up as wrong output and never as a hang.
 #include <hip/hip_runtime.h>
 
 #include <atomic>

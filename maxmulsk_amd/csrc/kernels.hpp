@@ -91,8 +91,9 @@ hipError_t launch_gemm(int mode, int cfg, const GemmParams& p, hipStream_t strea
 // multiply; same K slices, products and order as the direct tile. feed_mode():
 // 0 = direct tile kernels only, 1 = every 64 x 64 launch and every split-mode
 // 128 x 128 launch takes the feed, 2 = the feed where it measured faster (the
-// default). MAXMULSK_FEED=direct|
-// decoupled sets 0 or 1 at first use; set_feed_mode overrides.
+// default). Split-mode grouped launches take the feed under 1 and 2
+// (grouped_feed). MAXMULSK_FEED=direct|decoupled sets 0 or 1 at first use;
+// set_feed_mode overrides.
 int feed_mode();
 void set_feed_mode(int v);
 // Feed variant of a launch of geometry cfg (0: 64 x 64, 3: 128 x 128) for an
