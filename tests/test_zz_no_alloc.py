@@ -74,3 +74,16 @@ def test_noalloc_planner_and_limits():
     Ch = np.full((M, N), SENTINEL, np.float32)
     assert mm.sgemm_noalloc(A, B, Ch, M, K, N, 0, (64, 1)) == mm.ERR_UNSUPPORTED
     assert np.all(Ch == SENTINEL)
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_noalloc_whole_tile_split_call_needs_no_scratch(mode):
+    """A split-mode 256-wide call on whole 256 x 256 tiles splits its operands
+    on the fly, so it needs no workspace at all: through the static scratch it
+    runs as through a caller workspace, bit for bit, packing nothing."""
+    torch = torch_cuda()
+    M, K, N = 1024, 2048, 1024
+    assert mm.workspace_bytes(M, K, N, mode, (256, 1)) == 0
+    a, b, counters = _pair(torch, M, K, N, mode, (256, 1), 700 + mode)
+    assert np.array_equal(a, b), (mode, int((a != b).sum()))
+    assert counters["allocation"] == 0 and counters["pack"] == 0 and counters["compute"] == 1, counters
