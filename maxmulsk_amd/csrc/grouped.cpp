@@ -200,14 +200,16 @@ maxmulsk_status build(const maxmulsk_gemm* problems, size_t n, int mode, const m
     p->dev = dev;
     DeviceGuard guard(dev);
 
-    // Work list. In plain mode a problem on 128-row tiles whose last tile would
-    // hold at most 64 rows runs as two row ranges: the 128-row tiles of the
-    // first M - M % 128 rows and one 64-row tile for the rest (same plan, so the
-    // same K slices and the same products per element; only padding rows are
-    // dropped). On the 256-problem batch (M ~ U{1..512}) that cut the padded
-    // MFMA work from 22% to 12% and the plain launch from 32.6 to 30.8 ms. The
-    // split modes keep the padded tile: their eight-wave 64-row tiles split and
-    // stage more operand bytes per product, and split3 measured 1.6% slower.
+    // Work list. In plain mode, and in the split modes on the decoupled feed, a
+    // problem on 128-row tiles whose last tile would hold at most 64 rows runs
+    // as two row ranges: the 128-row tiles of the first M - M % 128 rows and one
+    // 64-row tile for the rest (same plan, so the same K slices and the same
+    // products per element; only padding rows are dropped). On the 256-problem
+    // batch (M ~ U{1..512}) that cut the padded MFMA work from 22% to 12%: the
+    // plain launch went from 32.6 to 30.8 ms, the feed's split3 from 12.15 to
+    // 11.88 ms and split6 from 17.84 to 17.55 ms. The direct split-mode kernel
+    // keeps the padded tile: its eight-wave 64-row tiles split and stage more
+    // operand bytes per product, and split3 measured 1.6% slower with ranges.
     // Row ranges are ordered by per-item cost (most expensive first).
     struct Part {
         const float* A;
@@ -228,7 +230,7 @@ maxmulsk_status build(const maxmulsk_gemm* problems, size_t n, int mode, const m
         if (!active[i]) continue;
         const size_t tail = g.M % 128;
         Resolved rm, rt;
-        if (mode == 0 && res[i].bm == 128 && g.M > 128 && tail >= 1 && tail <= 64 &&
+        if ((mode == 0 || feed) && res[i].bm == 128 && g.M > 128 && tail >= 1 && tail <= 64 &&
             resolve(g.M - tail, g.K, g.N, mode, used_plan[i], &rm, kGroupedMaxCfg, grouped_min_cfg(mode)) &&
             resolve(tail, g.K, g.N, mode, used_plan[i], &rt, kGroupedMaxCfg, grouped_min_cfg(mode)) && rm.bm == 128 &&
             rt.bm == 64 && rm.k_len == res[i].k_len && rt.k_len == res[i].k_len) {

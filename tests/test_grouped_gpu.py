@@ -72,9 +72,10 @@ def test_grouped_matches_oracle_and_standalone(mode):
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
 def test_grouped_tail_rows_match_standalone(mode):
-    """In plain mode, problems on 128-row tiles whose last tile holds at most
-    64 rows run as two row ranges (128-row tiles, then one 64-row tile); the
-    split modes keep the padded tile. Either way every row gets the standalone
+    """In plain mode and on the split modes' decoupled feed (their default),
+    problems on 128-row tiles whose last tile holds at most 64 rows run as two
+    row ranges (128-row tiles, then one 64-row tile); the direct split-mode
+    kernel keeps the padded tile. Either way every row gets the standalone
     call's K slices and products: bit-identical results, the bound, and
     nothing written outside C."""
     torch = torch_cuda()
