@@ -182,7 +182,11 @@ maxmulsk_status maxmulsk_sgemm_grouped(const maxmulsk_gemm* problems, size_t n,
  * maxmulsk_grouped_default_plan), or hold one plan
  * per problem. In a grouped launch a 256-wide tile on more than 128 rows runs
  * on the 128 x 256 geometry, and in the split modes a 64-wide tile runs on the
- * 64 x 128 geometry (same K slices). A plan owns its work counter and split-K
+ * 64 x 128 geometry (same K slices). A problem on 128-row tiles whose last tile
+ * holds at most 64 rows runs that tail on a 64-row tile (plain mode, and the
+ * split modes on the decoupled feed); the products per element are the same.
+ * The split modes run on the decoupled feed unless the feed setting is
+ * MAXMULSK_FEED_DIRECT when the plan is created. A plan owns its work counter and split-K
  * workspace: launches of one plan must not overlap on the device (launch it on
  * one stream, or order the streams); different plans are independent. */
 typedef struct maxmulsk_grouped_plan_s* maxmulsk_grouped_plan;
