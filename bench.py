@@ -68,7 +68,8 @@ def main() -> int:
     ap.add_argument("--only", default="", help="comma list of sections: main,narrow,feed,grouped (default all)")
     ap.add_argument("--modes", default="plain,split3,split6", help="modes timed at 4096^3")
     ap.add_argument("--grouped-classes", type=int, default=16,
-                    help="shape classes the grouped autotuner searches (heaviest first)")
+                    help="shape classes the grouped autotuner searches (heaviest first); -1: no tuning, "
+                         "the grouped rule's plans (for counter runs that should see only the timed launches)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 256 MiB device copy (reference for traffic counters)")
     args = ap.parse_args()
@@ -504,7 +505,10 @@ def bench_grouped(torch, mm, inputs, args, log, rounds) -> dict:
     for mode_name, mode in (("plain", 0), ("split3", 1)):
         rule = mm.grouped_plans(probs, mode)  # fresh process: the grouped cache is empty
         t = time.perf_counter()
-        tuned, tuned_ms = mm.grouped_autotune(probs, mode, max_classes=args.grouped_classes, reps=3)
+        if args.grouped_classes >= 0:
+            tuned, tuned_ms = mm.grouped_autotune(probs, mode, max_classes=args.grouped_classes, reps=3)
+        else:
+            tuned, tuned_ms = rule, None
         tune_s = time.perf_counter() - t
         used = mm.grouped_plans(probs, mode)
         assert used == tuned, "the grouped cache does not serve the autotuned plans"
