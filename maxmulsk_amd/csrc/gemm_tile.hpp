@@ -411,6 +411,59 @@ struct Tile {
         }
     }
 
+    // The epilogue through LDS (smem free again, every wave past the main
+    // loop's last barrier), 32 rows of the wave tile at a time: the
+    // accumulators are written to a padded row-major block per wave, then
+    // read back whole rows per instruction, so every global store covers
+    // complete 128-byte lines of C instead of 32-byte pieces of 32 rows. Same
+    // values as epilogue(). Used by the 256 x 256 tile, whose C writes it cuts
+    // from 1.46x to 1.00x the size of C (4096^3 split3: 93.2 -> 64.0 MiB) at
+    // the same speed.
+    __device__ __forceinline__ void epilogue_lds(float* __restrict__ C, long long M, long long N, long long ldc,
+                                                 long long m0, long long n0, char* smem) const {
+        constexpr int NW = WGM * WGN;
+        constexpr int PITCH = WTN + 4;  // floats; the 16-byte row shift keeps both patterns conflict-free
+        static_assert(NW * 32 * PITCH * 4 <= SMEM, "epilogue blocks fit in the tile's LDS");
+        constexpr int LPR = WTN / 4;    // lanes per row when reading
+        constexpr int RPI = 64 / LPR;   // rows per read instruction
+        static_assert(LPR * RPI == 64 && 32 % RPI == 0, "whole rows per read");
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int wm = wave / WGN, wn = wave % WGN;
+        const int r = lane & 31, h = lane >> 5;
+        float* blk = reinterpret_cast<float*>(smem) + wave * 32 * PITCH;
+        const int rr = lane / LPR, cc = (lane % LPR) * 4;
+#pragma unroll
+        for (int ti = 0; ti < TM; ++ti) {
+#pragma unroll
+            for (int tj = 0; tj < TN; ++tj)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const f32x4 v = {acc[ti][tj][4 * q + 0], acc[ti][tj][4 * q + 1], acc[ti][tj][4 * q + 2],
+                                     acc[ti][tj][4 * q + 3]};
+                    *reinterpret_cast<f32x4*>(blk + r * PITCH + tj * 32 + 8 * q + 4 * h) = v;
+                }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 32 / RPI; ++i) {
+                const int row = i * RPI + rr;
+                const long long m = m0 + wm * WTM + ti * 32 + row;
+                const long long n = n0 + wn * WTN + cc;
+                const f32x4 v = *reinterpret_cast<const f32x4*>(blk + row * PITCH + cc);
+                if (m < M) {
+                    float* crow = C + m * ldc;
+                    if (n + 3 < N) {
+                        *reinterpret_cast<f32x4u*>(crow + n) = v;
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            if (n + e < N) crow[n + e] = v[e];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
     // Main loop over the k-blocks of [kbeg, kend). ALL_FAST: every block is a
     // full block (vector loads, no bounds checks), so the steady-state
     // body is one basic block and the scheduler can interleave the MFMAs of
@@ -500,7 +553,10 @@ struct Tile {
             mainloop<true>(A, B, M, N, lda, ldb, m0, n0, kbeg, kend, full_mn, smem);
         else
             mainloop<false>(A, B, M, N, lda, ldb, m0, n0, kbeg, kend, full_mn, smem);
-        epilogue(C, M, N, ldc, m0, n0);
+        if constexpr (BM == 256 && BN == 256)
+            epilogue_lds(C, M, N, ldc, m0, n0, smem);
+        else
+            epilogue(C, M, N, ldc, m0, n0);
     }
 };
 
