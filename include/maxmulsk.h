@@ -383,6 +383,16 @@ const char* maxmulsk_last_error(void);
  * tile of `rows` rows (BM or BN) and depth bk, in the given mode; -1 when out
  * of range. */
 int maxmulsk_lds_chunk_offset(int mode, int op, int rows, int bk, int row, int c, int term);
+/* Host-side view of the epilogue staging of the 256 x 256 tile (a test hook
+ * with no counterpart in the reference): each wave passes 32 rows of its wave
+ * tile, wtn = 64 columns wide (the only width the tile has), through a
+ * row-major block of 32 x (wtn + 4) floats. _write: float offset in that block of lane `lane`'s
+ * 16-byte store of accumulator values 4q..4q+3 of fragment column tj (row
+ * lane % 32). _read: float offset of the four floats lane `lane` loads in read
+ * step i (0..wtn/8-1), whose row and first column of the wave tile go to
+ * *row and *col. Both return -1 when out of range. */
+int maxmulsk_lds_epilogue_write(int wtn, int lane, int tj, int q);
+int maxmulsk_lds_epilogue_read(int wtn, int lane, int i, int* row, int* col);
 /* Staging geometry of tile geometry cfg (0..maxmulsk_num_tile_geometries()-1)
  * in a mode, as one-call launches run it: out[0..5] = BM, BN, BK, threads per
  * workgroup, columns per B staging unit, waves (eight for the 128-row tiles of

@@ -116,6 +116,8 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "maxmulsk_device_count": (C.c_int, []),
     "maxmulsk_last_error": (C.c_char_p, []),
     "maxmulsk_lds_chunk_offset": (C.c_int, [C.c_int] * 7),
+    "maxmulsk_lds_epilogue_write": (C.c_int, [C.c_int] * 4),
+    "maxmulsk_lds_epilogue_read": (C.c_int, [C.c_int] * 3 + [C.POINTER(C.c_int)] * 2),
     "maxmulsk_num_tile_geometries": (C.c_int, []),
     "maxmulsk_tile_info": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int)]),
 }
@@ -296,6 +298,18 @@ def candidates() -> list[tuple[int, int]]:
 
 def lds_chunk_offset(mode: int, op: int, rows: int, bk: int, row: int, c: int, term: int) -> int:
     return lib().maxmulsk_lds_chunk_offset(mode, op, rows, bk, row, c, term)
+
+
+def lds_epilogue_write(wtn: int, lane: int, tj: int, q: int) -> int:
+    return lib().maxmulsk_lds_epilogue_write(wtn, lane, tj, q)
+
+
+def lds_epilogue_read(wtn: int, lane: int, i: int) -> tuple:
+    """(float offset, row, first column) of lane's read in step i; offset -1
+    when out of range."""
+    row, col = C.c_int(-1), C.c_int(-1)
+    off = lib().maxmulsk_lds_epilogue_read(wtn, lane, i, C.byref(row), C.byref(col))
+    return off, row.value, col.value
 
 
 def tile_info(cfg: int, mode: int) -> dict:

@@ -157,6 +157,20 @@ extern "C" int maxmulsk_lds_chunk_offset(int mode, int op, int rows, int bk, int
     return chunk_offset(mode, op, rows, bk, row, c, term);
 }
 
+extern "C" int maxmulsk_lds_epilogue_write(int wtn, int lane, int tj, int q) {
+    if (wtn != 64 || lane < 0 || lane >= 64 || tj < 0 || tj >= wtn / 32 || q < 0 || q >= 4)
+        return -1;
+    return epi_write_offset(wtn, lane, tj, q);
+}
+
+extern "C" int maxmulsk_lds_epilogue_read(int wtn, int lane, int i, int* row, int* col) {
+    if (wtn != 64 || lane < 0 || lane >= 64 || i < 0 || i >= wtn / 8 || !row || !col)
+        return -1;
+    *row = epi_read_row(wtn, lane, i);
+    *col = epi_read_col(wtn, lane);
+    return *row * epi_pitch(wtn) + *col;
+}
+
 extern "C" int maxmulsk_num_tile_geometries(void) { return kNumTileCfgs; }
 
 extern "C" int maxmulsk_tile_info(int cfg, int mode, int* out) {

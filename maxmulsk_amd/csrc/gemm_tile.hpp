@@ -422,16 +422,14 @@ struct Tile {
     __device__ __forceinline__ void epilogue_lds(float* __restrict__ C, long long M, long long N, long long ldc,
                                                  long long m0, long long n0, char* smem) const {
         constexpr int NW = WGM * WGN;
-        constexpr int PITCH = WTN + 4;  // floats; the 16-byte row shift keeps both patterns conflict-free
+        constexpr int PITCH = epi_pitch(WTN);  // floats (layout in lds_layout.hpp)
         static_assert(NW * 32 * PITCH * 4 <= SMEM, "epilogue blocks fit in the tile's LDS");
-        constexpr int LPR = WTN / 4;    // lanes per row when reading
-        constexpr int RPI = 64 / LPR;   // rows per read instruction
-        static_assert(LPR * RPI == 64 && 32 % RPI == 0, "whole rows per read");
+        constexpr int RPI = 256 / WTN;         // rows per read instruction (WTN / 4 lanes per row)
+        static_assert(WTN == 64, "the staging layout is conflict-free at 64-wide wave tiles");
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         const int wm = wave / WGN, wn = wave % WGN;
-        const int r = lane & 31, h = lane >> 5;
         float* blk = reinterpret_cast<float*>(smem) + wave * 32 * PITCH;
-        const int rr = lane / LPR, cc = (lane % LPR) * 4;
+        const int cc = epi_read_col(WTN, lane);
 #pragma unroll
         for (int ti = 0; ti < TM; ++ti) {
 #pragma unroll
@@ -440,12 +438,12 @@ struct Tile {
                 for (int q = 0; q < 4; ++q) {
                     const f32x4 v = {acc[ti][tj][4 * q + 0], acc[ti][tj][4 * q + 1], acc[ti][tj][4 * q + 2],
                                      acc[ti][tj][4 * q + 3]};
-                    *reinterpret_cast<f32x4*>(blk + r * PITCH + tj * 32 + 8 * q + 4 * h) = v;
+                    *reinterpret_cast<f32x4*>(blk + epi_write_offset(WTN, lane, tj, q)) = v;
                 }
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < 32 / RPI; ++i) {
-                const int row = i * RPI + rr;
+                const int row = epi_read_row(WTN, lane, i);
                 const long long m = m0 + wm * WTM + ti * 32 + row;
                 const long long n = n0 + wn * WTN + cc;
                 const f32x4 v = *reinterpret_cast<const f32x4*>(blk + row * PITCH + cc);

@@ -89,4 +89,20 @@ MMSK_HD constexpr int operand_bytes(int mode, int rows, int bk) {
     return mode_terms(mode) * rows * bk * (mode == 0 ? 4 : 2);
 }
 
+// Epilogue staging of the 256 x 256 tile: each wave passes 32 rows of its
+// wave tile (wtn = 64 columns) through a row-major block of 32 x epi_pitch(wtn)
+// floats. The 4-float pad shifts consecutive rows by 16 bytes, so a store (lane
+// l holds row l % 32) and a read (one whole 256-byte row per 16 lanes) each
+// touch every 16-byte slot once per cycle at this width.
+MMSK_HD constexpr int epi_pitch(int wtn) { return wtn + 4; }
+// Float offset in the wave's block of lane `lane`'s 16-byte store of the
+// accumulator values 4q..4q+3 of fragment column tj: row lane % 32, columns
+// tj * 32 + 8q + 4 * (lane / 32) .. +3.
+MMSK_HD constexpr int epi_write_offset(int wtn, int lane, int tj, int q) {
+    return (lane & 31) * epi_pitch(wtn) + tj * 32 + 8 * q + 4 * (lane >> 5);
+}
+// Row and first of the four columns that lane `lane` reads in read step i.
+MMSK_HD constexpr int epi_read_row(int wtn, int lane, int i) { return i * (256 / wtn) + lane / (wtn / 4); }
+MMSK_HD constexpr int epi_read_col(int wtn, int lane) { return (lane % (wtn / 4)) * 4; }
+
 }  // namespace mmsk

@@ -414,6 +414,42 @@ def test_lds_patterns_of_the_grouped_feed(mode, bm, bn, b_cols):
     _check_tile_lds_patterns(mode, bm, bn, 32, 256, b_cols)
 
 
+def test_lds_epilogue_of_the_256_tile():
+    """The 256 x 256 tile's LDS-staged epilogue (wave tiles 64 wide, the only
+    width the hook accepts): each wave's stores land inside its
+    32 x (wtn + 4) block without overlap; every element of the 32-row block is
+    read back exactly once, by the lane that stores it to the same row and
+    column of C as the register epilogue (accumulator value 4q + e of fragment
+    column tj in lane l: row l % 32, column tj * 32 + 8q + 4(l / 32) + e);
+    stores and reads are free of bank conflicts; eight waves' blocks fit in
+    the tile's 128 KiB of LDS."""
+    wtn = 64
+    block = 32 * (wtn + 4)
+    assert 8 * block * 4 <= 128 * 1024
+    held = {}
+    for tj in range(wtn // 32):
+        for q in range(4):
+            offs = [mm.lds_epilogue_write(wtn, lane, tj, q) for lane in range(64)]
+            assert _worst_conflict([4 * o for o in offs], 8, 128) == 1, (tj, q)
+            for lane, o in enumerate(offs):
+                assert 0 <= o and o + 4 <= block
+                for e in range(4):
+                    assert o + e not in held
+                    held[o + e] = (lane % 32, tj * 32 + 8 * q + 4 * (lane // 32) + e)
+    assert len(held) == 32 * wtn
+    seen = set()
+    for i in range(wtn // 8):
+        reads = [mm.lds_epilogue_read(wtn, lane, i) for lane in range(64)]
+        assert _worst_conflict([4 * o for o, _, _ in reads], 16, 256) == 1, i
+        for o, row, col in reads:
+            for e in range(4):
+                assert held[o + e] == (row, col + e)
+                seen.add((row, col + e))
+    assert len(seen) == 32 * wtn
+    assert mm.lds_epilogue_write(32, 0, 0, 0) == -1 and mm.lds_epilogue_write(wtn, 64, 0, 0) == -1
+    assert mm.lds_epilogue_read(wtn, 0, wtn // 8)[0] == -1
+
+
 def test_feed_variant_rule():
     """Which kernel a launch takes under each feed setting (pure host code):
     the automatic rule feeds launches of at most one workgroup per CU: 64 x 64
